@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark of the periodic Laplace KIFMM evaluation path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): N = 1,048,576 bodies uniform in the
+unit cube from a seeded PCG64, zero-sum uniform [-1, 1] charges, P = 8, ncrit = 64, periodic with
+3 image levels.  The unit cube maps to the reference frame by x - 0.5 with cycle = 1.
+
+One step = set_particles (adaptive octree + P2P/M2L lists) + evaluate (P2M, M2M, M2L, L2L, L2P,
+P2P, dipole correction) with the charges already in HBM.  The one-time operator precompute depends
+only on (P, images) and is excluded (reported separately).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "exafmm-rd_amd"))
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pts = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1.0, 1.0, n)
+    q -= q.mean()
+    q[-1] -= q.sum()          # exact zero net charge
+    return pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), q
+
+
+def hadamard_flops(plan):
+    """Algorithmic flops of one 8x8 Hadamard launch: 8 flops per complex MAC, 64 MACs per (pair, f)."""
+    import pyfmm3dpp
+    tar, src, off, img = plan.m2l_list()
+    pairs = int(np.sum(img == 0))
+    c = plan.counts()
+    return 8.0 * 64.0 * c["n_freq"] * pairs, pairs
+
+
+def cpu_baseline(x, y, z, q, P, ncrit, images, threads, budget_s):
+    """The CPU restatement (oracle/, C++/OpenMP) timed on this host's cores on the same inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    f = oracle.OracleFMM(P, ncrit, images, 1.0, dipole=1, threads=threads)
+    f.precompute()
+    t0 = time.perf_counter()
+    f.set_particles(x, y, z)
+    f.evaluate(q)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) / dt, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"one full step (tree+lists+all stages, precompute excluded) of the same N={len(x)} workload, "
+                      f"{dt:.1f} s", "seconds": round(dt, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--P", type=int, default=8)
+    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--images", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the evaluation path is single-device (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pyfmm3dpp
+
+    x, y, z, q = workload(args.n, args.seed)
+    plan = pyfmm3dpp.LaplaceFMM(args.P, args.ncrit, args.images, 1.0, dipole_correction=1, device=0)
+    t0 = time.perf_counter()
+    plan.precompute()
+    precompute_s = time.perf_counter() - t0
+    dev = torch.device("cuda", 0)
+    d_q = torch.from_numpy(q).to(dev)
+    d_phi = torch.empty(args.n, dtype=torch.float64, device=dev)
+    d_grad = torch.empty((args.n, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        plan.set_particles(x, y, z)
+        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stage_names = ["P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize", "evaluate"]
+    stages = {k: 0.0 for k in stage_names}
+    set_ms = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        plan.set_particles(x, y, z)
+        set_ms += (time.perf_counter() - ts) * 1e3
+        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
+        for k in stage_names:
+            stages[k] += plan.timing_ms(k)   # HIP events on the plan's stream (synchronises)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    stages = {k: round(v / args.steps, 4) for k, v in stages.items()}
+    value = args.n / (ms_per_step / 1e3)
+
+    flops, pairs = hadamard_flops(plan)
+    had_ms = stages["M2L_hadamard"]
+    achieved = flops / (had_ms / 1e3) / 1e12
+    roofline = {"kernel": "k_hadamard8 (M2L 8x8 Hadamard, FP64)", "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                "traffic": None, "flops_per_launch": flops, "pairs": pairs, "launch_ms": had_ms}
+
+    out = {
+        "metric": "particles/sec, N=1M P=8 periodic: mixed vs FP64 GPU at rel L2 err ≤1e-5",
+        "value": round(value, 1),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic Laplace KIFMM, N={args.n} uniform unit cube (PCG64 seed {args.seed}), "
+                               f"zero-sum uniform charges, P={args.P}, ncrit={args.ncrit}, images={args.images}",
+                   "path": "fp64", "tree_and_lists": "host (C++)", "parallelism": "single GPU"},
+        "breakdown_ms": dict(stages, set_particles=round(set_ms / args.steps, 3)),
+        "precompute_s": round(precompute_s, 3),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        out["cpu_baseline"] = cpu_baseline(x, y, z, q, args.P, args.ncrit, args.images, threads, 30)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

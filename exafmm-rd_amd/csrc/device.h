@@ -1,0 +1,65 @@
+// Device-side views and kernel launchers of the periodic FMM (float64 path).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fmm3dpp {
+
+// Structure-of-arrays view of the cells in HBM (BFS order + image cells).
+struct CellsDev {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* r;
+    const int32_t* depth;
+    const int32_t* boff;
+    const int32_t* bnum;
+};
+
+struct BodiesDev {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* q;
+};
+
+// Column-gathered GEMM: for every column c,
+//   Y[out_idx[c]][0:S] (+)= scale_c * sum_kb A[:, kb*S:(kb+1)*S] * X[in_idx[c*KB+kb]][0:S]
+// A is S x (KB*S) row-major; X and Y are [cell][S]; in_idx < 0 reads zeros;
+// scale_c = 2^(-depth[out or in cell]) when scale_pow2 != 0 (see launch_gemm_cols).
+struct GemmCols {
+    int S, KB, ncol, accumulate;
+    const double* A;
+    const double* X;
+    const int32_t* in_idx;
+    double* Y;
+    const int32_t* out_idx;
+    const int32_t* scale_depth;  // nullptr: scale 1; else scale_c = 0.5^scale_depth[c]
+};
+
+hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int n, hipStream_t st);
+hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
+                            BodiesDev b, double* pc, hipStream_t st);
+hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st);
+hipError_t launch_m2l_up(int P, int S, int nsrc, const int32_t* src_child, const int32_t* lattice,
+                         const double* twiddle, const double* qeq, double* Qk, hipStream_t st);
+hipError_t launch_hadamard8(int F, int ntar, int nsrc, const int32_t* slot_src_T, const double* gk8, const double* Qk,
+                            double* Pk, hipStream_t st);
+hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src, int ntar,
+                             int nsrc, const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);
+hipError_t launch_m2l_down(int P, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
+                           const int32_t* lattice, const double* twiddle, const double* Pk, double* pck,
+                           hipStream_t st);
+hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
+                           BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
+                           hipStream_t st);
+hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
+                      const int32_t* list_src, const double* list_shift, double* p, double* gx, double* gy,
+                      double* gz, hipStream_t st);
+hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 3 * nblocks */, int nblocks, hipStream_t st);
+hipError_t launch_finalize(int n, const int32_t* perm, const double* q, const double* p, const double* gx,
+                           const double* gy, const double* gz, const double* partial, int nblocks, int mode,
+                           double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
+
+}  // namespace fmm3dpp

@@ -1,0 +1,219 @@
+"""ctypes binding of libfmm3dpp.so (include/fmm3dpp.h), the MI355X periodic KIFMM.
+
+`LaplaceFMM` mirrors the reference's evaluate API (rtfmm::LaplaceFMM(bodies, args).solve(),
+3dpp/include/fmm.h:15-17): a plan is built once per configuration, `set_particles` builds the tree
+and lists, `solve`/`evaluate` return potential and gradient in input order.  The library must be
+built (`make -C exafmm-rd_amd/csrc`); nothing here falls back to another implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfmm3dpp.so")
+
+OK = 0
+ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_TREE, ERR_ALLOC, ERR_HIP = -1, -2, -3, -4, -5, -6
+PREC_FP64, PREC_MIXED = 0, 1
+DIPOLE_OFF, DIPOLE_REFERENCE = 0, 1
+DEVICE_NONE = -1
+
+# Every entry point include/fmm3dpp.h declares.
+EXPORTS = (
+    "fmm3dpp_plan_create", "fmm3dpp_plan_destroy", "fmm3dpp_last_error", "fmm3dpp_precompute",
+    "fmm3dpp_set_particles", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
+    "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
+    "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
+)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_int), ("ncrit", ctypes.c_int), ("images", ctypes.c_int), ("cycle", ctypes.c_double),
+                ("center", ctypes.c_double * 3), ("precision", ctypes.c_int), ("dipole_mode", ctypes.c_int),
+                ("divide_4pi", ctypes.c_int)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [("n_bodies", ctypes.c_int64), ("n_cells", ctypes.c_int64), ("n_tree_cells", ctypes.c_int64),
+                ("n_p2p", ctypes.c_int64), ("n_m2l", ctypes.c_int64), ("n_lost", ctypes.c_int64),
+                ("S", ctypes.c_int32), ("n_freq", ctypes.c_int32)]
+
+
+class FMMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fmm3dpp error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def lib():
+    """Load libfmm3dpp.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `make -C exafmm-rd_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.fmm3dpp_plan_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(vp)]
+        L.fmm3dpp_plan_destroy.argtypes = [vp]
+        L.fmm3dpp_last_error.argtypes = [vp]
+        L.fmm3dpp_last_error.restype = ctypes.c_char_p
+        L.fmm3dpp_precompute.argtypes = [vp]
+        L.fmm3dpp_set_particles.argtypes = [vp, ctypes.c_int64, _dp, _dp, _dp]
+        L.fmm3dpp_evaluate.argtypes = [vp, _dp, _dp, _dp]
+        L.fmm3dpp_evaluate_device.argtypes = [vp, vp, vp, vp]
+        L.fmm3dpp_set_stream.argtypes = [vp, vp]
+        L.fmm3dpp_get_counts.argtypes = [vp, ctypes.POINTER(Counts)]
+        L.fmm3dpp_get_cells.argtypes = [vp, _ip, _dp]
+        L.fmm3dpp_get_body_order.argtypes = [vp, _ip]
+        L.fmm3dpp_get_p2p.argtypes = [vp, _ip, _ip, _dp]
+        L.fmm3dpp_get_m2l.argtypes = [vp, _ip, _ip, _dp, _ip]
+        L.fmm3dpp_get_operator.argtypes = [vp, ctypes.c_char_p, _dp, ctypes.c_int64]
+        L.fmm3dpp_get_operator.restype = ctypes.c_int64
+        L.fmm3dpp_set_operator.argtypes = [vp, ctypes.c_char_p, _dp, ctypes.c_int64]
+        L.fmm3dpp_get_stage_vectors.argtypes = [vp, _dp, _dp]
+        L.fmm3dpp_timing_ms.argtypes = [vp, ctypes.c_char_p]
+        L.fmm3dpp_timing_ms.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class LaplaceFMM:
+    """Plan of the periodic Laplace KIFMM on one GPU (or structure-only with device=DEVICE_NONE).
+
+    Arguments follow the reference's Argument fields (3dpp/include/argument.h:14-44): P, ncrit,
+    images, cycle, dipole_correction, divide_4pi; the box is centred at `center` with half size cycle/2.
+    """
+
+    def __init__(self, P, ncrit, images, cycle, dipole_correction=1, divide_4pi=0, precision=PREC_FP64,
+                 device=0, center=(0.0, 0.0, 0.0)):
+        cfg = Config()
+        cfg.P, cfg.ncrit, cfg.images, cfg.cycle = P, ncrit, images, cycle
+        for k in range(3):
+            cfg.center[k] = center[k]
+        cfg.precision = precision
+        cfg.dipole_mode = DIPOLE_REFERENCE if dipole_correction else DIPOLE_OFF
+        cfg.divide_4pi = int(bool(divide_4pi))
+        h = ctypes.c_void_p()
+        rc = lib().fmm3dpp_plan_create(ctypes.byref(cfg), device, ctypes.byref(h))
+        if rc != OK:
+            raise FMMError(rc, "plan_create failed (invalid configuration or no device)")
+        self.h = h
+        self.P, self.ncrit, self.images, self.cycle = P, ncrit, images, cycle
+        self.device = device
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fmm3dpp_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise FMMError(rc, lib().fmm3dpp_last_error(self.h).decode())
+
+    def precompute(self):
+        self._check(lib().fmm3dpp_precompute(self.h))
+
+    def set_particles(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        self.n = len(x)
+        self._check(lib().fmm3dpp_set_particles(self.h, self.n, _d(x), _d(y), _d(z)))
+
+    def evaluate(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if len(q) != self.n:
+            raise ValueError("charge count differs from the particle count")
+        phi = np.empty(self.n)
+        grad = np.empty((self.n, 3))
+        self._check(lib().fmm3dpp_evaluate(self.h, _d(q), _d(phi), _d(grad)))
+        return phi, grad
+
+    def solve(self, x, y, z, q):
+        """set_particles + evaluate, like LaplaceFMM::solve (3dpp/src/fmm.cpp:13-81)."""
+        self.set_particles(x, y, z)
+        return self.evaluate(q)
+
+    def evaluate_device(self, q_ptr, phi_ptr, grad_ptr):
+        """Device pointers (ints), ordered on the plan's stream."""
+        self._check(lib().fmm3dpp_evaluate_device(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(phi_ptr),
+                                                   ctypes.c_void_p(grad_ptr)))
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().fmm3dpp_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def timing_ms(self, name):
+        return lib().fmm3dpp_timing_ms(self.h, name.encode())
+
+    # --- structure ---------------------------------------------------------
+    def counts(self):
+        c = Counts()
+        self._check(lib().fmm3dpp_get_counts(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in Counts._fields_}
+
+    def cells(self):
+        n = self.counts()["n_cells"]
+        ints = np.empty((n, 7), dtype=np.int32)
+        dbl = np.empty((n, 4))
+        self._check(lib().fmm3dpp_get_cells(self.h, _i(ints), _d(dbl)))
+        return ints, dbl
+
+    def body_order(self):
+        perm = np.empty(self.n, dtype=np.int32)
+        self._check(lib().fmm3dpp_get_body_order(self.h, _i(perm)))
+        return perm
+
+    def p2p_list(self):
+        m = self.counts()["n_p2p"]
+        tar, src = np.empty(m, dtype=np.int32), np.empty(m, dtype=np.int32)
+        off = np.empty((m, 3))
+        self._check(lib().fmm3dpp_get_p2p(self.h, _i(tar), _i(src), _d(off)))
+        return tar, src, off
+
+    def m2l_list(self):
+        m = self.counts()["n_m2l"]
+        tar, src, img = (np.empty(m, dtype=np.int32) for _ in range(3))
+        off = np.empty((m, 3))
+        self._check(lib().fmm3dpp_get_m2l(self.h, _i(tar), _i(src), _d(off), _i(img)))
+        return tar, src, off, img
+
+    # --- operators -----------------------------------------------------------
+    def get_operator(self, name):
+        n = lib().fmm3dpp_get_operator(self.h, name.encode(), None, 0)
+        if n < 0:
+            self._check(int(n))
+        out = np.empty(n)
+        lib().fmm3dpp_get_operator(self.h, name.encode(), _d(out), n)
+        return out
+
+    def set_operator(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._check(lib().fmm3dpp_set_operator(self.h, name.encode(), _d(arr), arr.size))
+
+    def stage_vectors(self):
+        c = self.counts()
+        qeq = np.empty((c["n_cells"], c["S"]))
+        pck = np.empty((c["n_cells"], c["S"]))
+        self._check(lib().fmm3dpp_get_stage_vectors(self.h, _d(qeq), _d(pck)))
+        return qeq, pck
+
+
+OPERATOR_NAMES = (["UT", "VSinv", "UT_l2p", "VSinv_l2p", "GK8"] + [f"M2M{o}" for o in range(8)]
+                  + [f"L2L{o}" for o in range(8)])
+IMAGE_OPERATOR_NAMES = [f"M2M_IMG{o}" for o in range(27)] + ["L2L_IMG"]

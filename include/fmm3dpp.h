@@ -1,0 +1,122 @@
+/*
+ * fmm3dpp — MI355X-native periodic kernel-independent FMM for the 3-D Laplace kernel.
+ *
+ * C-ABI of the evaluation path that replaces the `3dpp` C++ class API of mumairsadiq/exafmm-rd.
+ * Every entry point returns an int status (0 = OK, negative = error, see FMM3DPP_ERR_*) and never
+ * exits the process; fmm3dpp_last_error() gives the message.  The caller owns every host array
+ * (structure-of-arrays, float64); the plan owns every device allocation.  One plan is not
+ * thread-safe; distinct plans may be used concurrently.
+ *
+ * Reference interfaces replaced (paths under the reference's 3dpp/ directory):
+ *   fmm3dpp_plan_create      <- rtfmm::LaplaceFMM::LaplaceFMM(const Bodies3&, const Argument&)
+ *                               (include/fmm.h:15; src/fmm.cpp:7-11) and Argument (src/argument.cpp:7-70)
+ *   fmm3dpp_precompute       <- LaplaceKernel::precompute + precompute_m2l (src/kernel.cpp:1638-1850)
+ *   fmm3dpp_set_particles    <- Tree::build (src/tree.cpp:10-20,102-188) + Traverser::traverse
+ *                               (src/traverser.cpp:19-68)
+ *   fmm3dpp_evaluate         <- LaplaceFMM::solve stages P2M..P2P, dipole_correction, scale_bodies,
+ *                               sort_bodies_by_idx (src/fmm.cpp:50-80)
+ *   fmm3dpp_get_cells/_p2p/_m2l <- Tree::get_cells, Traverser::get_map(P2P), get_M2L_parent_map
+ *                               (include/tree.h:69; include/traverser.h:50-54)
+ *   fmm3dpp_get/set_operator <- the private operator members of LaplaceKernel (include/kernel.h:197-229)
+ */
+#ifndef FMM3DPP_H
+#define FMM3DPP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMM3DPP_OK 0
+#define FMM3DPP_ERR_ARG (-1)      /* invalid argument or configuration                          */
+#define FMM3DPP_ERR_DEVICE (-2)   /* no usable GPU, or the plan was created without one          */
+#define FMM3DPP_ERR_STATE (-3)    /* call out of order (e.g. evaluate before set_particles)      */
+#define FMM3DPP_ERR_TREE (-4)     /* tree cannot be built (e.g. > ncrit coincident bodies)       */
+#define FMM3DPP_ERR_ALLOC (-5)    /* host or device allocation failed                            */
+#define FMM3DPP_ERR_HIP (-6)      /* a HIP runtime call failed                                   */
+
+#define FMM3DPP_PREC_FP64 0       /* every stage in float64                                       */
+#define FMM3DPP_PREC_MIXED 1      /* float32 near field and M2L, float64 operator applies/lattice */
+
+#define FMM3DPP_DIPOLE_OFF 0      /* no periodic dipole correction                                */
+#define FMM3DPP_DIPOLE_REFERENCE 1 /* the reference's correction (src/kernel.cpp:1580-1596)      */
+
+#define FMM3DPP_DEVICE_NONE (-1)  /* structure-only plan: tree, lists and operators on the host; */
+                                  /* evaluate returns FMM3DPP_ERR_DEVICE                          */
+
+typedef struct fmm3dpp_config {
+    int P;              /* points per surface edge, 2..16  (Argument::P, -P)                 */
+    int ncrit;          /* max bodies per leaf, >= 1       (Argument::ncrit, -m)             */
+    int images;         /* periodic image levels, 0..20    (Argument::images, -i); 0 = free space */
+    double cycle;       /* period (box edge)               (Argument::cycle, -c)             */
+    double center[3];   /* box centre                      (Argument::x; reference uses 0)   */
+    int precision;      /* FMM3DPP_PREC_*                                                     */
+    int dipole_mode;    /* FMM3DPP_DIPOLE_*                (Argument::dipole_correction)     */
+    int divide_4pi;     /* scale outputs by 1/(4 pi)       (Argument::divide_4pi)            */
+} fmm3dpp_config;
+
+typedef struct fmm3dpp_plan fmm3dpp_plan;
+
+/* Plan creation binds the calling thread's HIP device `device` (or FMM3DPP_DEVICE_NONE). */
+int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** out);
+int fmm3dpp_plan_destroy(fmm3dpp_plan* plan);
+const char* fmm3dpp_last_error(const fmm3dpp_plan* plan);
+
+/* One-time operator precompute for (P, images, cycle); implied by the first evaluate. */
+int fmm3dpp_precompute(fmm3dpp_plan* plan);
+
+/* Builds the adaptive octree and the interaction lists for n bodies (host arrays, input order). */
+int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const double* y, const double* z);
+
+/* Potential phi and gradient grad(phi) (the reference's `f`) for charges q, all in input order.
+ * grad is n x 3, row-major (gx, gy, gz per body).  Host arrays. */
+int fmm3dpp_evaluate(fmm3dpp_plan* plan, const double* q, double* phi, double* grad);
+
+/* Same as fmm3dpp_evaluate with device pointers, ordered on the plan's stream (no host sync
+ * other than what the tree build needs).  d_grad is n x 3 row-major. */
+int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_q, double* d_phi, double* d_grad);
+
+/* Sets the HIP stream (hipStream_t) the plan launches on; NULL = the default stream. */
+int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* hip_stream);
+
+/* ---- structure, for bit-exact comparison with the reference's tree and lists ---- */
+typedef struct fmm3dpp_counts {
+    int64_t n_bodies;
+    int64_t n_cells;        /* tree cells + image cells                                   */
+    int64_t n_tree_cells;   /* cells produced by Tree::build                              */
+    int64_t n_p2p;          /* rows of the P2P map (target, source, offset)               */
+    int64_t n_m2l;          /* rows of the M2L parent map (target, source, offset, image) */
+    int64_t n_lost;         /* well-separated leaf pairs with no operator (reference prints "here") */
+    int32_t S;              /* surface points 6(P-1)^2+2                                  */
+    int32_t n_freq;         /* (2P-1)^2 (P) complex frequencies per child                 */
+} fmm3dpp_counts;
+
+int fmm3dpp_get_counts(const fmm3dpp_plan* plan, fmm3dpp_counts* out);
+/* ints: 7 per cell (idx, octant, depth, child_offset, child_count, body_offset, body_count);
+ * dbl: 4 per cell (r, x, y, z).  BFS order with the image cells appended, as Tree/Traverser. */
+int fmm3dpp_get_cells(const fmm3dpp_plan* plan, int32_t* ints, double* dbl);
+/* Tree order -> input index of every body. */
+int fmm3dpp_get_body_order(const fmm3dpp_plan* plan, int32_t* perm);
+int fmm3dpp_get_p2p(const fmm3dpp_plan* plan, int32_t* tar, int32_t* src, double* off3);
+int fmm3dpp_get_m2l(const fmm3dpp_plan* plan, int32_t* tar, int32_t* src, double* off3, int32_t* is_image);
+
+/* ---- operators (names as the reference members: UT, VSinv, UT_l2p, VSinv_l2p, M2M0..7,
+ *      M2M_IMG0..26, L2L0..7, L2L_IMG, GK8 [26][F][64][2], GK27 [26][F][28][2]) ----
+ * get returns the element count (call with out = NULL to query); set replaces one operator
+ * and re-derives the fused device forms from the current factors. */
+int64_t fmm3dpp_get_operator(fmm3dpp_plan* plan, const char* name, double* out, int64_t cap);
+int fmm3dpp_set_operator(fmm3dpp_plan* plan, const char* name, const double* in, int64_t n);
+
+/* Per-cell equivalent densities and check potentials [n_cells][S] after the last evaluate. */
+int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_check);
+
+/* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
+ * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
+ * L2L, L2P, P2P, finalize, evaluate.  -1 if unknown. */
+double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMM3DPP_H */

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <queue>
 #include <set>
 #include <string>
@@ -714,6 +715,7 @@ static void stage_m2m(Plan& pl, int mind, int maxd) {
 // kernel.cpp:1140-1397 (m2l_fft_precompute_t) with hadamard_8x8 (1012-1082) and hadamard_1x27 (1084-1138)
 static void stage_m2l(Plan& pl) {
     int S = pl.S, N = pl.N, N3 = pl.N3, Nh = pl.Nh, F = pl.F, P = pl.P;
+    double ts0 = now();
     std::vector<int> tars;
     std::set<int> srcset;
     for (int t = 0; t < (int)pl.m2l.size(); t++) {
@@ -742,9 +744,20 @@ static void stage_m2l(Plan& pl) {
         }
     }
     size_t fsz = (size_t)F * 8;  // complex entries per parent
-    std::vector<cplx> Qk(srcs.size() * fsz), Pk(tars.size() * fsz, cplx(0, 0));
+    double ts1 = now();
+    // like the reference (kernel.cpp:1263-1283): Qk is fully written by the up pass, Pk is zeroed in parallel
+    struct Free { void operator()(cplx* p) const { std::free(p); } };
+    std::unique_ptr<cplx[], Free> Qk_mem(static_cast<cplx*>(std::malloc(srcs.size() * fsz * sizeof(cplx) + 64)));
+    std::unique_ptr<cplx[], Free> Pk_mem(static_cast<cplx*>(std::malloc(tars.size() * fsz * sizeof(cplx) + 64)));
+    cplx* Qk = Qk_mem.get();
+    cplx* Pk = Pk_mem.get();
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)tars.size(); t++)
+        for (size_t e = 0; e < fsz; e++) Pk[t * fsz + e] = cplx(0, 0);
     Dft dft(N);
     double t0 = now();
+    pl.timers["m2l_setup"] = ts1 - ts0;
+    pl.timers["m2l_alloc"] = t0 - ts1;
     // up (kernel.cpp:1286-1328)
 #pragma omp parallel for schedule(dynamic, 2)
     for (int si = 0; si < (int)srcs.size(); si++) {
