@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -375,14 +376,22 @@ int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const 
     if (!plan) return FMM3DPP_ERR_ARG;
     if (!x || !y || !z) FAIL(FMM3DPP_ERR_ARG, "null position array");
     plan->particles = false;
+    auto clk = [] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
     int rc = build_structure(plan->cfg, n, x, y, z, plan->st, plan->err);
     if (rc) return rc;
+    double t0 = clk();
     derive_indices(plan);
+    double t1 = clk();
     if (plan->device >= 0) {
         HIPCK(hipSetDevice(plan->device));
         rc = upload_structure(plan);
         if (rc) return rc;
     }
+    plan->host_ms["tree"] = plan->st.ms_tree;
+    plan->host_ms["lists_p2p"] = plan->st.ms_p2p;
+    plan->host_ms["lists_m2l"] = plan->st.ms_m2l;
+    plan->host_ms["indices"] = t1 - t0;
+    plan->host_ms["upload"] = clk() - t1;
     plan->particles = true;
     return FMM3DPP_OK;
 }
@@ -620,7 +629,10 @@ int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_che
 }
 
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
-    if (!plan || !name || plan->device < 0 || !plan->ev_recorded) return -1.0;
+    if (!plan || !name) return -1.0;
+    auto hit = plan->host_ms.find(name);
+    if (hit != plan->host_ms.end()) return hit->second;
+    if (plan->device < 0 || !plan->ev_recorded) return -1.0;
     auto* p = const_cast<fmm3dpp_plan*>(plan);
     if (hipEventSynchronize(p->ev[kStages]) != hipSuccess) return -1.0;
     std::string s(name);

@@ -10,6 +10,7 @@
 // neighbour search for the M2L parent map.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -73,6 +74,8 @@ int build_structure(const fmm3dpp_config& cfg, int64_t n, const double* X, const
     if (n < 1 || n > (int64_t)INT32_MAX) { err = "number of bodies must be in [1, 2^31)"; return FMM3DPP_ERR_ARG; }
     const double r0 = cfg.cycle / 2;
     const double* c0 = cfg.center;
+    auto clk = [] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    const double t_start = clk();
     for (int64_t i = 0; i < n; i++) {
         if (!std::isfinite(X[i]) || !std::isfinite(Y[i]) || !std::isfinite(Z[i])) { err = "non-finite body position"; return FMM3DPP_ERR_ARG; }
     }
@@ -184,6 +187,8 @@ int build_structure(const fmm3dpp_config& cfg, int64_t n, const double* X, const
     }
     st.min_depth = images >= 1 ? -images : 0;
 
+    const double t_tree = clk();
+    st.ms_tree = t_tree - t_start;
     // 7. P2P map: level-synchronous restatement of horizontal_origin (traverser.cpp:70-148)
     const std::vector<CellRec>& C = st.cells;
     std::vector<std::array<double, 3>> offs;
@@ -234,6 +239,8 @@ int build_structure(const fmm3dpp_config& cfg, int64_t n, const double* X, const
     }
     std::stable_sort(st.p2p.begin(), st.p2p.end(), [](const P2PRow& a, const P2PRow& b) { return a.tar < b.tar; });
 
+    const double t_p2p = clk();
+    st.ms_p2p = t_p2p - t_tree;
     // 8. M2L parent map (traverser.cpp:234-282 and 43-64) on the integer lattice of each level:
     //    source = target + delta (delta in {-1,0,1}^3), wrapped with offset o*cycle when periodic.
     st.m2l.clear();
@@ -283,6 +290,7 @@ int build_structure(const fmm3dpp_config& cfg, int64_t n, const double* X, const
                 for (int px = -1; px <= 1; px++)
                     if (px || py || pz) st.m2l.push_back({ci, ci, {px * w, py * w, pz * w}, 1, pattern_of(px, py, pz)});
     }
+    st.ms_m2l = clk() - t_p2p;
     return FMM3DPP_OK;
 }
 

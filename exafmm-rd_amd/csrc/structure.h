@@ -28,6 +28,7 @@ struct Structure {
     std::vector<P2PRow> p2p;         // P2P map rows (targets may be non-leaf)
     std::vector<M2LRow> m2l;         // M2L parent map rows
     int64_t lost = 0;                // well-separated leaf pairs the traversal drops
+    double ms_tree = 0, ms_p2p = 0, ms_m2l = 0;  // host build timings of the last call
 };
 
 int build_structure(const fmm3dpp_config& cfg, int64_t n, const double* x, const double* y, const double* z,

@@ -113,7 +113,8 @@ int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_che
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
- * L2L, L2P, P2P, finalize, evaluate.  -1 if unknown. */
+ * L2L, L2P, P2P, finalize, evaluate; host wall times of the last set_particles: tree, lists_p2p,
+ * lists_m2l, indices, upload.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);
 
 #ifdef __cplusplus
