@@ -24,24 +24,30 @@ struct BodiesDev {
     const double* q;
 };
 
-// Column-gathered GEMM: for every column c,
-//   Y[out_idx[c]][0:S] (+)= scale_c * sum_kb A[:, kb*S:(kb+1)*S] * X[in_idx[c*KB+kb]][0:S]
-// A is S x (KB*S) row-major; X and Y are [cell][S]; in_idx < 0 reads zeros;
-// scale_c = 2^(-depth[out or in cell]) when scale_pow2 != 0 (see launch_gemm_cols).
+// Column-gathered batched GEMM, batch z = blockIdx.z:
+//   Y_z[out_idx[z*idx_zstride + c]][0:S] (+)= scale_c * A_z * X[in_idx[z*idx_zstride + c]][0:S]
+// with A_z = A + z*a_zstride (S x S row-major), X and Y_z = Y + z*y_zstride laid out [vector][S];
+// in_idx < 0 reads zeros, out_idx < 0 skips the column; scale_c = 0.5^scale_depth[c] if given.
 struct GemmCols {
-    int S, KB, ncol, accumulate;
+    int S, ncol, nz, accumulate;
     const double* A;
+    int64_t a_zstride;
     const double* X;
     const int32_t* in_idx;
-    double* Y;
     const int32_t* out_idx;
-    const int32_t* scale_depth;  // nullptr: scale 1; else scale_c = 0.5^scale_depth[c]
+    int32_t idx_zstride;
+    double* Y;
+    int64_t y_zstride;
+    const int32_t* scale_depth;
 };
 
 hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int n, hipStream_t st);
 hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                             BodiesDev b, double* pc, hipStream_t st);
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st);
+// Y[out[c]][r] = sum_{z < nz} part[z][c][r]  (fixed order: deterministic M2M child sum)
+hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
+                               hipStream_t st);
 hipError_t launch_m2l_up(int P, int S, int nsrc, const int32_t* src_child, const int32_t* lattice,
                          const double* twiddle, const double* qeq, double* Qk, hipStream_t st);
 hipError_t launch_hadamard8(int F, int ntar, int nsrc, const int32_t* slot_src_T, const double* gk8, const double* Qk,

@@ -73,37 +73,46 @@ __global__ __launch_bounds__(256) void k_p2m_check(int S, const double* __restri
     }
 }
 
-// Column-gathered GEMM for the operator applies: P2M/L2P pseudo-inverse (src/kernel.cpp:369-370,
-// 1465-1467), M2M with the 8 child operators stacked along K (395-404), L2L per octant (1419-1428)
-// and the image-level M2M/L2L (434-442, 1430-1435).  64x64 output tile per workgroup, 4x4 per thread.
+// Column-gathered batched GEMM for the operator applies: P2M/L2P pseudo-inverse factors
+// (src/kernel.cpp:369-370, 1465-1467), M2M per child octant (395-404), L2L per child octant
+// (1419-1428) and the image-level M2M/L2L (434-442, 1430-1435).  64x64 output tile per workgroup,
+// 4x4 per thread, K in steps of 16 with the next tile prefetched into registers.
 constexpr int TM = 64, TN = 64, TK = 16;
 __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
     __shared__ double As[TK][TM + 2];
     __shared__ double Bs[TK][TN + 2];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int row0 = blockIdx.x * TM, col0 = blockIdx.y * TN;
-    const int S = g.S, K = g.KB * g.S;
+    const int row0 = blockIdx.x * TM, col0 = blockIdx.y * TN, z = blockIdx.z;
+    const int S = g.S, K = g.S;
+    const double* A = g.A + (int64_t)z * g.a_zstride;
+    const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
+    const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
+    double* Y = g.Y + (int64_t)z * g.y_zstride;
+    const int lr = tid >> 2, lk = (tid & 3) * 4;
+    const int gr = row0 + lr, gc = col0 + lr;
+    const int src = gc < g.ncol ? in_idx[gc] : -1;
+    const double* arow = gr < S ? A + (int64_t)gr * K : nullptr;
+    const double* bcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int gk = k0 + lk + u;
+            ra[u] = (arow && gk < K) ? arow[gk] : 0.0;
+            rb[u] = (bcol && gk < K) ? bcol[gk] : 0.0;
+        }
+    };
     double acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    const int lr = tid >> 2, lk = (tid & 3) * 4;
-    const int gr = row0 + lr, gc = col0 + lr;
+    fetch(0);
     for (int k0 = 0; k0 < K; k0 += TK) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            int gk = k0 + lk + u;
-            As[lk + u][lr] = (gr < S && gk < K) ? g.A[(size_t)gr * K + gk] : 0.0;
-            double v = 0.0;
-            if (gc < g.ncol && gk < K) {
-                int kb = gk / S, kl = gk - kb * S;
-                int idx = g.in_idx[(size_t)gc * g.KB + kb];
-                if (idx >= 0) v = g.X[(size_t)idx * S + kl];
-            }
-            Bs[lk + u][lr] = v;
-        }
+        for (int u = 0; u < 4; u++) { As[lk + u][lr] = ra[u]; Bs[lk + u][lr] = rb[u]; }
         __syncthreads();
+        if (k0 + TK < K) fetch(k0 + TK);
 #pragma unroll
         for (int kk = 0; kk < TK; kk++) {
             double a[4], bb[4];
@@ -122,17 +131,28 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
     for (int j = 0; j < 4; j++) {
         int c = col0 + tx * 4 + j;
         if (c >= g.ncol) continue;
-        int oi = g.out_idx[c];
+        int oi = out_idx[c];
+        if (oi < 0) continue;
         double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             int r = row0 + ty * 4 + i;
             if (r >= S) continue;
             double v = sc * acc[i][j];
-            double* y = &g.Y[(size_t)oi * S + r];
+            double* y = &Y[(int64_t)oi * S + r];
             *y = g.accumulate ? *y + v : v;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, const double* __restrict__ part,
+                                                      const int32_t* __restrict__ out, double* __restrict__ Y) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)ncol * S) return;
+    const int c = (int)(e / S), r = (int)(e - (int64_t)c * S);
+    double a = 0;
+    for (int z = 0; z < nz; z++) a += part[((int64_t)z * ncol + c) * S + r];
+    Y[(int64_t)out[c] * S + r] = a;
 }
 
 // M2L up (src/kernel.cpp:1286-1328): scatter each child's S equivalent charges on the P^3 corner of
@@ -515,8 +535,16 @@ hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell
 
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
     if (g.ncol <= 0) return hipSuccess;
-    dim3 grid((g.S + TM - 1) / TM, (g.ncol + TN - 1) / TN);
+    dim3 grid((g.S + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
     k_gemm_cols<<<grid, 256, 0, st>>>(g);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
+                               hipStream_t st) {
+    if (ncol <= 0) return hipSuccess;
+    int64_t tot = (int64_t)ncol * S;
+    k_sum_partials<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, ncol, nz, part, out, Y);
     return hipGetLastError();
 }
 

@@ -67,7 +67,8 @@ struct fmm3dpp_plan {
     std::vector<int32_t> m2m_in, m2m_out;          // per level, 8 children per parent
     std::vector<GemmLevel> m2m_levels;
     std::vector<int32_t> l2l_in, l2l_out;
-    std::vector<std::pair<GemmLevel, int>> l2l_groups;  // (columns, octant)
+    std::vector<GemmLevel> l2l_levels;
+    int max_level_cols = 0;
     std::vector<std::array<int32_t, 2>> img_m2m;   // (parent c(-m), child), deepest first
     std::vector<std::array<int32_t, 2>> img_l2l;   // (parent, child), outermost first
     std::vector<int32_t> src_cells, tar_cells, src_child, tar_child, slot_src_T, img_src_T, img_tar;
@@ -79,11 +80,11 @@ struct fmm3dpp_plan {
     // device buffers
     DBuf d_x, d_y, d_z, d_perm, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
     DBuf d_cx, d_cy, d_cz, d_cr, d_depth, d_boff, d_bnum;
-    DBuf d_leaf_cell, d_leaf_depth, d_iota, d_pcleaf, d_pcleaf2, d_qeq, d_pck;
+    DBuf d_leaf_cell, d_leaf_depth, d_iota, d_iota8, d_pcleaf, d_pcleaf2, d_qeq, d_pck;
     DBuf d_m2m_in, d_m2m_out, d_l2l_in, d_l2l_out, d_img_a, d_img_b;
     DBuf d_src_child, d_tar_child, d_slot_src_T, d_img_src_T, d_img_tar, d_tar_scale, d_Qk, d_Pk;
-    DBuf d_p2p_off, d_p2p_src, d_p2p_shift, d_partial;
-    DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_cat, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk27, o_usurf, o_lattice, o_tw;
+    DBuf d_p2p_off, d_p2p_src, d_p2p_shift, d_partial, d_m2m_part;
+    DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_all, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk27, o_usurf, o_lattice, o_tw;
 
     CellsDev cells_dev() const {
         return {d_cx.as<double>(), d_cy.as<double>(), d_cz.as<double>(), d_cr.as<double>(), d_depth.as<int32_t>(),
@@ -129,34 +130,45 @@ void derive_indices(fmm3dpp_plan* plan) {
     plan->leaf_depth.resize(nleaf);
     plan->iota.resize(nleaf);
     for (int i = 0; i < nleaf; i++) { plan->leaf_depth[i] = C[plan->leaf_cell[i]].depth; plan->iota[i] = i; }
-    // M2M per level, deepest first: parent columns with 8 child slots by octant
+    // M2M and L2L per level: parent columns, child slots octant-major ([octant][column])
     plan->m2m_in.clear(); plan->m2m_out.clear(); plan->m2m_levels.clear();
-    plan->l2l_in.clear(); plan->l2l_out.clear(); plan->l2l_groups.clear();
+    plan->l2l_in.clear(); plan->l2l_out.clear(); plan->l2l_levels.clear();
     std::vector<std::vector<int32_t>> by_depth(S.max_depth + 1);
     for (int32_t c = 0; c < S.ntree; c++) if (C[c].cnum > 0) by_depth[C[c].depth].push_back(c);
+    plan->max_level_cols = 0;
     for (int d = S.max_depth - 1; d >= 0; d--) {
-        GemmLevel lv{(int)by_depth[d].size(), plan->m2m_in.size(), plan->m2m_out.size()};
-        for (int32_t par : by_depth[d]) {
-            int32_t slots[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-            for (int k = 0; k < C[par].cnum; k++) slots[C[C[par].coff + k].octant] = C[par].coff + k;
-            for (int o = 0; o < 8; o++) plan->m2m_in.push_back(slots[o]);
-            plan->m2m_out.push_back(par);
-        }
-        if (lv.ncol > 0) plan->m2m_levels.push_back(lv);
+        const auto& pars = by_depth[d];
+        int ncol = (int)pars.size();
+        if (ncol == 0) continue;
+        plan->max_level_cols = std::max(plan->max_level_cols, ncol);
+        GemmLevel lv{ncol, plan->m2m_in.size(), plan->m2m_out.size()};
+        std::vector<int32_t> slots((size_t)8 * ncol, -1);
+        for (int c = 0; c < ncol; c++)
+            for (int k = 0; k < C[pars[c]].cnum; k++) {
+                int32_t ch = C[pars[c]].coff + k;
+                slots[(size_t)C[ch].octant * ncol + c] = ch;
+            }
+        plan->m2m_in.insert(plan->m2m_in.end(), slots.begin(), slots.end());
+        plan->m2m_out.insert(plan->m2m_out.end(), pars.begin(), pars.end());
+        plan->m2m_levels.push_back(lv);
     }
-    for (int d = 0; d < S.max_depth; d++)
-        for (int o = 0; o < 8; o++) {
-            GemmLevel lv{0, plan->l2l_in.size(), plan->l2l_out.size()};
-            for (int32_t par : by_depth[d])
-                for (int k = 0; k < C[par].cnum; k++) {
-                    int32_t ch = C[par].coff + k;
-                    if (C[ch].octant != o) continue;
-                    plan->l2l_in.push_back(par);
-                    plan->l2l_out.push_back(ch);
-                    lv.ncol++;
-                }
-            if (lv.ncol > 0) plan->l2l_groups.push_back({lv, o});
+    for (int d = 0; d < S.max_depth; d++) {
+        const auto& pars = by_depth[d];
+        int ncol = (int)pars.size();
+        if (ncol == 0) continue;
+        GemmLevel lv{ncol, plan->l2l_in.size(), plan->l2l_out.size()};
+        std::vector<int32_t> slots((size_t)8 * ncol, -1), src((size_t)8 * ncol);
+        for (int c = 0; c < ncol; c++) {
+            for (int o = 0; o < 8; o++) src[(size_t)o * ncol + c] = pars[c];
+            for (int k = 0; k < C[pars[c]].cnum; k++) {
+                int32_t ch = C[pars[c]].coff + k;
+                slots[(size_t)C[ch].octant * ncol + c] = ch;
+            }
         }
+        plan->l2l_in.insert(plan->l2l_in.end(), src.begin(), src.end());
+        plan->l2l_out.insert(plan->l2l_out.end(), slots.begin(), slots.end());
+        plan->l2l_levels.push_back(lv);
+    }
     plan->img_m2m.clear(); plan->img_l2l.clear();
     const int images = plan->cfg.images;
     // M2M image levels run for depths -1 .. -(images-1) (fmm.cpp:114: depth > min depth)
@@ -233,14 +245,15 @@ void derive_indices(fmm3dpp_plan* plan) {
 int upload_operators(fmm3dpp_plan* plan) {
     hipStream_t s = plan->stream;
     const Operators& o = plan->ops;
-    std::vector<double> l2l_all;
+    std::vector<double> l2l_all, m2m_all;
     for (int k = 0; k < 8; k++) l2l_all.insert(l2l_all.end(), o.l2l[k].begin(), o.l2l[k].end());
+    for (int k = 0; k < 8; k++) m2m_all.insert(m2m_all.end(), o.m2m[k].begin(), o.m2m[k].end());
     std::vector<double> zeros((size_t)o.S * o.S, 0.0);
     HIPCK(upload(plan->o_ut, o.UT, s));
     HIPCK(upload(plan->o_vs, o.VSinv, s));
     HIPCK(upload(plan->o_utl, o.UT_l2p, s));
     HIPCK(upload(plan->o_vsl, o.VSinv_l2p, s));
-    HIPCK(upload(plan->o_m2m_cat, o.m2m_cat, s));
+    HIPCK(upload(plan->o_m2m_all, m2m_all, s));
     HIPCK(upload(plan->o_l2l, l2l_all, s));
     HIPCK(upload(plan->o_m2m_img, o.images > 0 ? o.m2m_img_sum : zeros, s));
     HIPCK(upload(plan->o_l2l_img, o.images > 0 ? o.l2l_img : zeros, s));
@@ -275,6 +288,12 @@ int upload_structure(fmm3dpp_plan* plan) {
     HIPCK(upload(plan->d_leaf_cell, plan->leaf_cell, s));
     HIPCK(upload(plan->d_leaf_depth, plan->leaf_depth, s));
     HIPCK(upload(plan->d_iota, plan->iota, s));
+    {   // 0 .. 8*widest-1: M2M partial of (octant z, column c) goes to row z*ncol + c of the scratch
+        int w = std::max(1, plan->max_level_cols);
+        std::vector<int32_t> io8((size_t)8 * w);
+        for (size_t i = 0; i < io8.size(); i++) io8[i] = (int32_t)i;
+        HIPCK(upload(plan->d_iota8, io8, s));
+    }
     HIPCK(upload(plan->d_m2m_in, plan->m2m_in, s)); HIPCK(upload(plan->d_m2m_out, plan->m2m_out, s));
     HIPCK(upload(plan->d_l2l_in, plan->l2l_in, s)); HIPCK(upload(plan->d_l2l_out, plan->l2l_out, s));
     std::vector<int32_t> ia, ib;
@@ -299,6 +318,7 @@ int upload_structure(fmm3dpp_plan* plan) {
     HIPCK(plan->d_Qk.ensure((size_t)F * plan->src_cells.size() * 8 * 16));
     HIPCK(plan->d_Pk.ensure((size_t)F * plan->tar_cells.size() * 8 * 16));
     HIPCK(plan->d_partial.ensure(256 * 3 * 8));
+    HIPCK(plan->d_m2m_part.ensure((size_t)8 * std::max(1, plan->max_level_cols) * Sn * 8));
     HIPCK(hipStreamSynchronize(s));
     return FMM3DPP_OK;
 }
@@ -420,23 +440,28 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     HIPCK(launch_p2m_check(Sn, plan->o_usurf.as<double>(), plan->d_leaf_cell.as<int32_t>(), nleaf, cells, bod,
                            plan->d_pcleaf.as<double>(), s));
     {   // q_equiv = 0.5^depth * VSinv * (UT * p_check)   (kernel.cpp:369-370)
-        GemmCols g1{Sn, 1, nleaf, 0, plan->o_ut.as<double>(), plan->d_pcleaf.as<double>(), plan->d_iota.as<int32_t>(),
-                    plan->d_pcleaf2.as<double>(), plan->d_iota.as<int32_t>(), nullptr};
+        const int32_t* iota = plan->d_iota.as<int32_t>();
+        GemmCols g1{Sn, nleaf, 1, 0, plan->o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
+                    plan->d_pcleaf2.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, 1, nleaf, 0, plan->o_vs.as<double>(), plan->d_pcleaf2.as<double>(), plan->d_iota.as<int32_t>(),
-                    qeq, plan->d_leaf_cell.as<int32_t>(), plan->d_leaf_depth.as<int32_t>()};
+        GemmCols g2{Sn, nleaf, 1, 0, plan->o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota,
+                    plan->d_leaf_cell.as<int32_t>(), 0, qeq, 0, plan->d_leaf_depth.as<int32_t>()};
         HIPCK(launch_gemm_cols(g2, s));
     }
     HIPCK(hipEventRecord(plan->ev[1], s));
-    // M2M (fmm.cpp:111-150)
+    // M2M (fmm.cpp:111-150): per level, one GEMM per child octant into partials, then the child sum
+    const size_t SS = (size_t)Sn * Sn;
     for (const auto& lv : plan->m2m_levels) {
-        GemmCols g{Sn, 8, lv.ncol, 0, plan->o_m2m_cat.as<double>(), qeq, plan->d_m2m_in.as<int32_t>() + lv.in_off,
-                   qeq, plan->d_m2m_out.as<int32_t>() + lv.out_off, nullptr};
+        GemmCols g{Sn, lv.ncol, 8, 0, plan->o_m2m_all.as<double>(), (int64_t)SS, qeq,
+                   plan->d_m2m_in.as<int32_t>() + lv.in_off, plan->d_iota8.as<int32_t>(), lv.ncol,
+                   plan->d_m2m_part.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(g, s));
+        HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(),
+                                  plan->d_m2m_out.as<int32_t>() + lv.out_off, qeq, s));
     }
     for (size_t i = 0; i < plan->img_m2m.size(); i++) {
-        GemmCols g{Sn, 1, 1, 0, plan->o_m2m_img.as<double>(), qeq, plan->d_img_b.as<int32_t>() + i, qeq,
-                   plan->d_img_a.as<int32_t>() + i, nullptr};
+        GemmCols g{Sn, 1, 1, 0, plan->o_m2m_img.as<double>(), 0, qeq, plan->d_img_b.as<int32_t>() + i,
+                   plan->d_img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
         HIPCK(launch_gemm_cols(g, s));
     }
     HIPCK(hipEventRecord(plan->ev[2], s));
@@ -454,27 +479,28 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     HIPCK(launch_m2l_down(o.P, Sn, ntar, plan->d_tar_child.as<int32_t>(), plan->d_tar_scale.as<double>(),
                           plan->o_lattice.as<int32_t>(), plan->o_tw.as<double>(), plan->d_Pk.as<double>(), pck, s));
     HIPCK(hipEventRecord(plan->ev[5], s));
-    // L2L (fmm.cpp:195-229): image levels from the outermost image, then the tree levels
+    // L2L (fmm.cpp:195-229): image levels from the outermost image, then one launch per tree level
     const size_t nimgm2m = plan->img_m2m.size();
     for (size_t i = 0; i < plan->img_l2l.size(); i++) {
-        GemmCols g{Sn, 1, 1, 1, plan->o_l2l_img.as<double>(), pck, plan->d_img_a.as<int32_t>() + nimgm2m + i, pck,
-                   plan->d_img_b.as<int32_t>() + nimgm2m + i, nullptr};
+        GemmCols g{Sn, 1, 1, 1, plan->o_l2l_img.as<double>(), 0, pck, plan->d_img_a.as<int32_t>() + nimgm2m + i,
+                   plan->d_img_b.as<int32_t>() + nimgm2m + i, 0, pck, 0, nullptr};
         HIPCK(launch_gemm_cols(g, s));
     }
-    for (const auto& gr : plan->l2l_groups) {
-        GemmCols g{Sn, 1, gr.first.ncol, 1, plan->o_l2l.as<double>() + (size_t)gr.second * Sn * Sn, pck,
-                   plan->d_l2l_in.as<int32_t>() + gr.first.in_off, pck, plan->d_l2l_out.as<int32_t>() + gr.first.out_off,
-                   nullptr};
+    for (const auto& lv : plan->l2l_levels) {
+        GemmCols g{Sn, lv.ncol, 8, 1, plan->o_l2l.as<double>(), (int64_t)SS, pck,
+                   plan->d_l2l_in.as<int32_t>() + lv.in_off, plan->d_l2l_out.as<int32_t>() + lv.out_off, lv.ncol,
+                   pck, 0, nullptr};
         HIPCK(launch_gemm_cols(g, s));
     }
     HIPCK(hipEventRecord(plan->ev[6], s));
     // L2P (fmm.cpp:231-257)
     {   // q_down = 0.5^depth * VSinv_l2p * (UT_l2p * p_check)   (kernel.cpp:1465-1467)
-        GemmCols g1{Sn, 1, nleaf, 0, plan->o_utl.as<double>(), pck, plan->d_leaf_cell.as<int32_t>(),
-                    plan->d_pcleaf2.as<double>(), plan->d_iota.as<int32_t>(), nullptr};
+        const int32_t* iota = plan->d_iota.as<int32_t>();
+        GemmCols g1{Sn, nleaf, 1, 0, plan->o_utl.as<double>(), 0, pck, plan->d_leaf_cell.as<int32_t>(), iota, 0,
+                    plan->d_pcleaf2.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, 1, nleaf, 0, plan->o_vsl.as<double>(), plan->d_pcleaf2.as<double>(), plan->d_iota.as<int32_t>(),
-                    plan->d_pcleaf.as<double>(), plan->d_iota.as<int32_t>(), plan->d_leaf_depth.as<int32_t>()};
+        GemmCols g2{Sn, nleaf, 1, 0, plan->o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
+                    plan->d_pcleaf.as<double>(), 0, plan->d_leaf_depth.as<int32_t>()};
         HIPCK(launch_gemm_cols(g2, s));
     }
     HIPCK(launch_l2p_eval(Sn, plan->o_usurf.as<double>(), plan->d_leaf_cell.as<int32_t>(), nleaf, cells, bod,
