@@ -5,8 +5,8 @@ Workload (the configuration the headline metric is quoted on): N = 1,048,576 bod
 unit cube from a seeded PCG64, zero-sum uniform [-1, 1] charges, P = 8, ncrit = 64, periodic with
 3 image levels.  The unit cube maps to the reference frame by x - 0.5 with cycle = 1.
 
-One step = set_particles (adaptive octree + P2P/M2L lists) + evaluate (P2M, M2M, M2L, L2L, L2P,
-P2P, dipole correction) with the charges already in HBM.  The one-time operator precompute depends
+One step = set_particles_device (adaptive octree + P2P/M2L lists, built on the GPU) + evaluate (P2M,
+M2M, M2L, L2L, L2P, P2P, dipole correction) with positions and charges already in HBM.  The one-time operator precompute depends
 only on (P, images) and is excluded (reported separately).  Prints one JSON line.
 """
 import argparse
@@ -84,13 +84,14 @@ def main():
     plan.precompute()
     precompute_s = time.perf_counter() - t0
     dev = torch.device("cuda", 0)
+    d_x, d_y, d_z = (torch.from_numpy(a).to(dev) for a in (x, y, z))
     d_q = torch.from_numpy(q).to(dev)
     d_phi = torch.empty(args.n, dtype=torch.float64, device=dev)
     d_grad = torch.empty((args.n, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
     def step():
-        plan.set_particles(x, y, z)
+        plan.set_particles_device(args.n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
         plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
 
     for _ in range(args.warmup):
@@ -103,7 +104,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        plan.set_particles(x, y, z)
+        plan.set_particles_device(args.n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
         set_ms += (time.perf_counter() - ts) * 1e3
         plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
         for k in stage_names:
@@ -136,7 +137,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"periodic Laplace KIFMM, N={args.n} uniform unit cube (PCG64 seed {args.seed}), "
                                f"zero-sum uniform charges, P={args.P}, ncrit={args.ncrit}, images={args.images}",
-                   "path": "fp64", "tree_and_lists": "host (C++)", "parallelism": "single GPU"},
+                   "path": "fp64", "tree_and_lists": "GPU", "inputs": "positions and charges resident in HBM",
+                   "parallelism": "single GPU"},
         "breakdown_ms": dict(stages, set_particles=round(set_ms / args.steps, 3)),
         "precompute_s": round(precompute_s, 3),
         "roofline": roofline,

@@ -45,7 +45,7 @@ hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int
 hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                             BodiesDev b, double* pc, hipStream_t st);
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st);
-// Y[out[c]][r] = sum_{z < nz} part[z][c][r]  (fixed order: deterministic M2M child sum)
+// Y[out[c]][r] = sum_{z < nz} part[z][c][r] for out[c] >= 0 (fixed order: deterministic M2M child sum)
 hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
                                hipStream_t st);
 hipError_t launch_m2l_up(int P, int S, int nsrc, const int32_t* src_child, const int32_t* lattice,
@@ -60,9 +60,10 @@ hipError_t launch_m2l_down(int P, int S, int ntar, const int32_t* tar_child, con
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
                            hipStream_t st);
+// list_ent[e] = source cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle)
 hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
-                      const int32_t* list_src, const double* list_shift, double* p, double* gx, double* gy,
-                      double* gz, hipStream_t st);
+                      const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
+                      hipStream_t st);
 hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 3 * nblocks */, int nblocks, hipStream_t st);
 hipError_t launch_finalize(int n, const int32_t* perm, const double* q, const double* p, const double* gx,
                            const double* gy, const double* gz, const double* partial, int nblocks, int mode,

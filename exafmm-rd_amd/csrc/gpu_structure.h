@@ -1,0 +1,53 @@
+// Tree and interaction lists built on the GPU (bit-exact with the reference's Tree/Traverser).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dbuf.h"
+#include "fmm3dpp.h"
+#include "structure.h"
+
+namespace fmm3dpp {
+
+struct LevelTable { int ncol; int64_t in_off, out_off; };
+
+struct GpuStructure {
+    // sizes (host mirror)
+    int64_t n = 0;
+    int ntree = 0, ncells = 0, nleaf = 0, maxd = 0, nl = 0, nimg = 0, ntar = 0;
+    int64_t n_rows = 0, n_entries = 0, lost = 0;
+    int max_level_cols = 1;
+    std::vector<int32_t> level_start;          // tree levels 0..maxd, plus end
+    std::vector<LevelTable> m2m_levels;        // deepest first
+    std::vector<LevelTable> l2l_levels;        // root first
+    std::vector<std::array<int32_t, 2>> img_m2m, img_l2l;  // (parent, child) image-cell pairs
+    double ms_build = 0;
+
+    // bodies in tree order
+    DBuf x, y, z, perm;
+    // cells in BFS order followed by the image cells
+    DBuf prefix, cx, cy, cz, cr, depth, octant, coff, cnum, boff, bnum;
+    size_t cell_cap = 0;
+    // leaves in body order
+    DBuf leaf_cell, leaf_depth, iota;
+    // operator-apply column tables
+    DBuf m2m_in, m2m_out, l2l_in, l2l_out, img_a, img_b;
+    // M2L Hadamard tables (sources = targets = non-leaf cells, then image cells)
+    DBuf nl_idx, nl_cells, child_slots, tar_scale, slot_src_T, img_src_T, img_tar;
+    // P2P rows (target, source, offset index) and per-leaf CSR (source << 5 | offset index)
+    DBuf rows_tc, rows_sc, rows_oi, p2p_off, p2p_ent;
+    size_t rows_cap = 0;
+    // temporaries
+    DBuf key, key_s, idx0, perm1, tkey, tkey_s, ldep, head, rank, nchild, cstart, scan32, packed, scan64,
+        ekey, eval, ekey_s, eval_s, cnt64, flags, lvl_info, small, temp;
+    DBuf f_tc[2], f_sc[2], f_tcp[2], f_scp[2], f_oi[2];
+    size_t f_cap[2] = {0, 0};
+
+    int build(const fmm3dpp_config& cfg, int64_t n, const double* dx, const double* dy, const double* dz,
+              hipStream_t st, std::string& err);
+    // Host copies for the structure queries of the C-ABI.
+    int export_structure(const fmm3dpp_config& cfg, Structure& out, hipStream_t st, std::string& err);
+};
+
+}  // namespace fmm3dpp

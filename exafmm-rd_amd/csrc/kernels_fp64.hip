@@ -150,6 +150,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, c
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= (int64_t)ncol * S) return;
     const int c = (int)(e / S), r = (int)(e - (int64_t)c * S);
+    if (out[c] < 0) return;   // leaf column of an adaptive level: no M2M
     double a = 0;
     for (int z = 0; z < nz; z++) a += part[((int64_t)z * ncol + c) * S + r];
     Y[(int64_t)out[c] * S + r] = a;
@@ -414,9 +415,9 @@ __global__ __launch_bounds__(256) void k_l2p_eval(int S, const double* __restric
 // P2P near field (src/kernel.cpp:258-311): one wave per target leaf, one target body per lane;
 // source coordinates are wave-uniform so they arrive through scalar loads.
 __global__ __launch_bounds__(64) void k_p2p(const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
-                                            const int64_t* __restrict__ list_off, const int32_t* __restrict__ list_src,
-                                            const double* __restrict__ list_shift, double* __restrict__ p,
-                                            double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz) {
+                                            const int64_t* __restrict__ list_off, const int32_t* __restrict__ list_ent,
+                                            double cycle, double* __restrict__ p, double* __restrict__ gx,
+                                            double* __restrict__ gy, double* __restrict__ gz) {
     const int li = blockIdx.x, tid = threadIdx.x;
     const int c = leaf_cell[li];
     const int boff = cells.boff[c], bnum = cells.bnum[c];
@@ -428,8 +429,8 @@ __global__ __launch_bounds__(64) void k_p2p(const int32_t* __restrict__ leaf_cel
         const double xt = b.x[bi], yt = b.y[bi], zt = b.z[bi];
         double ap = 0, ax = 0, ay = 0, az = 0;
         for (int64_t e = e0; e < e1; e++) {
-            const int sc = list_src[e];
-            const double ox = list_shift[3 * e], oy = list_shift[3 * e + 1], oz = list_shift[3 * e + 2];
+            const int ent = list_ent[e], sc = ent >> 5, oi = ent & 31;   // source cell, periodic offset index
+            const double ox = (oi % 3 - 1) * cycle, oy = ((oi / 3) % 3 - 1) * cycle, oz = (oi / 9 - 1) * cycle;
             const int sb = cells.boff[sc], sn = cells.bnum[sc];
             for (int j = sb; j < sb + sn; j++) {
                 double dx = xt - (b.x[j] + ox), dy = yt - (b.y[j] + oy), dz = zt - (b.z[j] + oz);
@@ -634,10 +635,10 @@ hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell,
 }
 
 hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
-                      const int32_t* list_src, const double* list_shift, double* p, double* gx, double* gy,
-                      double* gz, hipStream_t st) {
+                      const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
+                      hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
-    k_p2p<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_src, list_shift, p, gx, gy, gz);
+    k_p2p<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_ent, cycle, p, gx, gy, gz);
     return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ DEVICE_NONE = -1
 # Every entry point include/fmm3dpp.h declares.
 EXPORTS = (
     "fmm3dpp_plan_create", "fmm3dpp_plan_destroy", "fmm3dpp_last_error", "fmm3dpp_precompute",
-    "fmm3dpp_set_particles", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
+    "fmm3dpp_set_particles", "fmm3dpp_set_particles_device", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
 )
@@ -65,6 +65,7 @@ def lib():
         L.fmm3dpp_last_error.restype = ctypes.c_char_p
         L.fmm3dpp_precompute.argtypes = [vp]
         L.fmm3dpp_set_particles.argtypes = [vp, ctypes.c_int64, _dp, _dp, _dp]
+        L.fmm3dpp_set_particles_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
         L.fmm3dpp_evaluate.argtypes = [vp, _dp, _dp, _dp]
         L.fmm3dpp_evaluate_device.argtypes = [vp, vp, vp, vp]
         L.fmm3dpp_set_stream.argtypes = [vp, vp]
@@ -135,6 +136,12 @@ class LaplaceFMM:
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
         self.n = len(x)
         self._check(lib().fmm3dpp_set_particles(self.h, self.n, _d(x), _d(y), _d(z)))
+
+    def set_particles_device(self, n, x_ptr, y_ptr, z_ptr):
+        """Positions already in HBM (device pointers as ints); the tree and lists are built on the GPU."""
+        self.n = int(n)
+        self._check(lib().fmm3dpp_set_particles_device(self.h, self.n, ctypes.c_void_p(x_ptr),
+                                                       ctypes.c_void_p(y_ptr), ctypes.c_void_p(z_ptr)))
 
     def evaluate(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -66,8 +66,14 @@ const char* fmm3dpp_last_error(const fmm3dpp_plan* plan);
 /* One-time operator precompute for (P, images, cycle); implied by the first evaluate. */
 int fmm3dpp_precompute(fmm3dpp_plan* plan);
 
-/* Builds the adaptive octree and the interaction lists for n bodies (host arrays, input order). */
+/* Builds the adaptive octree and the interaction lists for n bodies (host arrays, input order).
+ * Device plans build on the GPU; structure-only plans (FMM3DPP_DEVICE_NONE) on the host. */
 int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const double* y, const double* z);
+
+/* Same as fmm3dpp_set_particles with device arrays (positions already in HBM); the whole tree and
+ * list build runs on the GPU.  Ordered on the plan's stream; returns after the build completed. */
+int fmm3dpp_set_particles_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y,
+                                 const double* d_z);
 
 /* Potential phi and gradient grad(phi) (the reference's `f`) for charges q, all in input order.
  * grad is n x 3, row-major (gx, gy, gz per body).  Host arrays. */
@@ -113,8 +119,8 @@ int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_che
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
- * L2L, L2P, P2P, finalize, evaluate; host wall times of the last set_particles: tree, lists_p2p,
- * lists_m2l, indices, upload.  -1 if unknown. */
+ * L2L, L2P, P2P, finalize, evaluate; host wall times: tree_and_lists (last set_particles),
+ * precompute.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);
 
 #ifdef __cplusplus
