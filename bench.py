@@ -6,8 +6,11 @@ unit cube from a seeded PCG64, zero-sum uniform [-1, 1] charges, P = 8, ncrit = 
 3 image levels.  The unit cube maps to the reference frame by x - 0.5 with cycle = 1.
 
 One step = set_particles_device (adaptive octree + P2P/M2L lists, built on the GPU) + evaluate (P2M,
-M2M, M2L, L2L, L2P, P2P, dipole correction) with positions and charges already in HBM.  The one-time operator precompute depends
-only on (P, images) and is excluded (reported separately).  Prints one JSON line.
+M2M, M2L, L2L, L2P, P2P, dipole correction) with positions and charges already in HBM.  The one-time
+operator precompute depends only on (P, images) and is excluded (reported separately).  Both
+precision paths are timed the same way on the same inputs; `value` is the mixed path's throughput
+(`--path fp64` reports the float64 path instead), and the line carries the mixed-vs-float64
+difference of potential and gradient.  Prints one JSON line.
 """
 import argparse
 import json
@@ -21,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "exafmm-rd_amd"))
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
@@ -57,6 +61,55 @@ def cpu_baseline(x, y, z, q, P, ncrit, images, threads, budget_s):
                       f"{dt:.1f} s", "seconds": round(dt, 3)}
 
 
+STAGES = ["P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize", "evaluate"]
+
+
+def run_path(pyfmm3dpp, torch, args, precision, d_x, d_y, d_z, d_q, images=None, steps=None, warmup=None):
+    """Warm up, then time exactly `steps` steps (set_particles_device + evaluate_device) of one
+    precision path; per-stage times come from HIP events on the plan's stream."""
+    n = args.n
+    images = args.images if images is None else images
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    plan = pyfmm3dpp.LaplaceFMM(args.P, args.ncrit, images, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_PHYSICAL,
+                                precision=precision, device=0)
+    t0 = time.perf_counter()
+    plan.precompute()
+    precompute_s = time.perf_counter() - t0
+    dev = d_x.device
+    d_phi = torch.empty(n, dtype=torch.float64, device=dev)
+    d_grad = torch.empty((n, 3), dtype=torch.float64, device=dev)
+
+    def step():
+        plan.set_particles_device(n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
+        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stages = {k: 0.0 for k in STAGES}
+    set_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        plan.set_particles_device(n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
+        set_ms += (time.perf_counter() - ts) * 1e3
+        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
+        for k in STAGES:
+            stages[k] += plan.timing_ms(k)   # HIP events on the plan's stream (synchronises)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / max(steps, 1)
+    stages = {k: round(v / max(steps, 1), 4) for k, v in stages.items()}
+    stages["set_particles"] = round(set_ms / max(steps, 1), 3)
+    return {"plan": plan, "ms_per_step": ms, "stages": stages, "precompute_s": precompute_s,
+            "phi": d_phi, "grad": d_grad}
+
+
+def rel_l2(torch, a, b):
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +120,10 @@ def main():
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--images", type=int, default=3)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--path", choices=["mixed", "fp64"], default="mixed",
+                    help="path whose throughput is the reported value (the other is timed beside it)")
+    ap.add_argument("--single-path", action="store_true",
+                    help="time only the --path path, no accuracy legs (for kernel profiles of one path)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -79,68 +136,90 @@ def main():
     import pyfmm3dpp
 
     x, y, z, q = workload(args.n, args.seed)
-    plan = pyfmm3dpp.LaplaceFMM(args.P, args.ncrit, args.images, 1.0, dipole_correction=1, device=0)
-    t0 = time.perf_counter()
-    plan.precompute()
-    precompute_s = time.perf_counter() - t0
     dev = torch.device("cuda", 0)
     d_x, d_y, d_z = (torch.from_numpy(a).to(dev) for a in (x, y, z))
     d_q = torch.from_numpy(q).to(dev)
-    d_phi = torch.empty(args.n, dtype=torch.float64, device=dev)
-    d_grad = torch.empty((args.n, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    def step():
-        plan.set_particles_device(args.n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
-        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stage_names = ["P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize", "evaluate"]
-    stages = {k: 0.0 for k in stage_names}
-    set_ms = 0.0
-    torch.cuda.synchronize()
+    if args.single_path:
+        prec = pyfmm3dpp.PREC_MIXED if args.path == "mixed" else pyfmm3dpp.PREC_FP64
+        r = run_path(pyfmm3dpp, torch, args, prec, d_x, d_y, d_z, d_q)
+        print(json.dumps({"path": args.path, "ms_per_step": round(r["ms_per_step"], 3),
+                          "particles_per_s": round(args.n / (r["ms_per_step"] / 1e3), 1),
+                          "breakdown_ms": r["stages"]}), flush=True)
+        return
+    r64 = run_path(pyfmm3dpp, torch, args, pyfmm3dpp.PREC_FP64, d_x, d_y, d_z, d_q)
+    rmx = run_path(pyfmm3dpp, torch, args, pyfmm3dpp.PREC_MIXED, d_x, d_y, d_z, d_q)
+    main_run = rmx if args.path == "mixed" else r64
+    value = args.n / (main_run["ms_per_step"] / 1e3)
+    v64 = args.n / (r64["ms_per_step"] / 1e3)
+    vmx = args.n / (rmx["ms_per_step"] / 1e3)
+    err = {"phi_rel_l2_mixed_vs_fp64": rel_l2(torch, rmx["phi"], r64["phi"]),
+           "grad_rel_l2_mixed_vs_fp64": rel_l2(torch, rmx["grad"], r64["grad"])}
+    # accuracy against a float64 Ewald sum (alpha L = 12, |k_i| <= 23, r_c = L/2) on 4096 sampled targets
+    tar = np.sort(np.random.Generator(np.random.PCG64(args.seed + 1)).choice(args.n, 4096, replace=False))
+    d_tar = torch.from_numpy(tar.astype(np.int32)).to(dev)
+    d_pe = torch.empty(len(tar), dtype=torch.float64, device=dev)
+    d_ge = torch.empty((len(tar), 3), dtype=torch.float64, device=dev)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        plan.set_particles_device(args.n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr())
-        set_ms += (time.perf_counter() - ts) * 1e3
-        plan.evaluate_device(d_q.data_ptr(), d_phi.data_ptr(), d_grad.data_ptr())
-        for k in stage_names:
-            stages[k] += plan.timing_ms(k)   # HIP events on the plan's stream (synchronises)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    stages = {k: round(v / args.steps, 4) for k, v in stages.items()}
-    value = args.n / (ms_per_step / 1e3)
+    rc = pyfmm3dpp.lib().fmm3dpp_ewald_device(None, args.n, d_x.data_ptr(), d_y.data_ptr(), d_z.data_ptr(),
+                                              d_q.data_ptr(), len(tar), d_tar.data_ptr(), 1.0, 23, 12.0, 0.5,
+                                              d_pe.data_ptr(), d_ge.data_ptr())
+    ewald_s = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"fmm3dpp_ewald_device failed ({rc})")
+    idx = torch.from_numpy(tar).to(dev)
+    def phi_err(phi):   # periodic potentials agree up to an additive constant: also report with it removed
+        d = phi[idx] - d_pe
+        return rel_l2(torch, phi[idx], d_pe), rel_l2(torch, phi[idx] - d.mean(), d_pe)
 
-    flops, pairs = hadamard_flops(plan)
-    had_ms = stages["M2L_hadamard"]
+    for name, r in (("fp64", r64), ("mixed", rmx)):
+        err[f"phi_rel_l2_{name}_vs_ewald"], err[f"phi_rel_l2_{name}_vs_ewald_mod_const"] = phi_err(r["phi"])
+        err[f"grad_rel_l2_{name}_vs_ewald"] = rel_l2(torch, r["grad"][idx], d_ge)
+    # the same at 5 image levels, where the lattice remainder drops below the FMM's own error
+    for name, prec in (("fp64", pyfmm3dpp.PREC_FP64), ("mixed", pyfmm3dpp.PREC_MIXED)):
+        r5 = run_path(pyfmm3dpp, torch, args, prec, d_x, d_y, d_z, d_q, images=5, steps=1, warmup=1)
+        err[f"phi_rel_l2_{name}_vs_ewald_images5"], err[f"phi_rel_l2_{name}_vs_ewald_images5_mod_const"] = \
+            phi_err(r5["phi"])
+        err[f"grad_rel_l2_{name}_vs_ewald_images5"] = rel_l2(torch, r5["grad"][idx], d_ge)
+        if name == "mixed":
+            err["mixed_ms_per_step_images5"] = round(r5["ms_per_step"], 3)
+    err["ewald_seconds"] = round(ewald_s, 3)
+
+    flops, pairs = hadamard_flops(main_run["plan"])
+    had_ms = main_run["stages"]["M2L_hadamard"]
     achieved = flops / (had_ms / 1e3) / 1e12
-    roofline = {"kernel": "k_hadamard8 (M2L 8x8 Hadamard, FP64)", "bound": "mfma", "achieved": round(achieved, 3),
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                "traffic": None, "flops_per_launch": flops, "pairs": pairs, "launch_ms": had_ms}
+    mixed = args.path == "mixed"
+    peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    roofline = {"kernel": "k_hadamard8_mfma<%s> (M2L 8x8 Hadamard on the matrix cores)" % ("float" if mixed else "double"),
+                "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "flops_per_launch": flops, "pairs": pairs,
+                "launch_ms": had_ms}
 
     out = {
-        "metric": "particles/sec, N=1M P=8 periodic: mixed vs FP64 GPU at rel L2 err ≤1e-5",
+        "metric": "particles/sec, N=1M P=8 periodic: mixed vs FP64 GPU at rel L2 err \u22641e-5",
         "value": round(value, 1),
         "unit": "particles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(main_run["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"periodic Laplace KIFMM, N={args.n} uniform unit cube (PCG64 seed {args.seed}), "
                                f"zero-sum uniform charges, P={args.P}, ncrit={args.ncrit}, images={args.images}",
-                   "path": "fp64", "tree_and_lists": "GPU", "inputs": "positions and charges resident in HBM",
+                   "path": args.path, "tree_and_lists": "GPU", "inputs": "positions and charges resident in HBM",
                    "parallelism": "single GPU"},
-        "breakdown_ms": dict(stages, set_particles=round(set_ms / args.steps, 3)),
-        "precompute_s": round(precompute_s, 3),
+        "fp64_particles_per_s": round(v64, 1),
+        "mixed_particles_per_s": round(vmx, 1),
+        "mixed_speedup_vs_fp64": round(vmx / v64, 3),
+        "accuracy": {k: float("%.3e" % v) for k, v in err.items()},
+        "gauge": "infinite-lattice (DIPOLE_PHYSICAL), compared with the tin-foil Ewald sum",
+        "breakdown_ms": {"mixed": rmx["stages"], "fp64": r64["stages"]},
+        "precompute_s": round(main_run["precompute_s"], 3),
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -340,27 +340,36 @@ __global__ void k_frontier_emit(int64_t nitems, const int32_t* __restrict__ ftc,
     }
 }
 
-// P2P rows -> (leaf rank, source << 5 | offset index); rows with a non-leaf target cover every leaf below it.
-__global__ void k_p2p_count(int64_t nrows, const int32_t* __restrict__ rtc, const int32_t* __restrict__ boff,
-                            const int32_t* __restrict__ bnum, const int32_t* __restrict__ rank_incl,
-                            int64_t* __restrict__ cnt) {
+// P2P rows -> CSR entries (target leaf rank, source leaf cell << 5 | offset index): one entry per
+// (target leaf, source leaf) under the row's two cells.  Adaptive trees have rows with non-leaf cells on either side (traverser.cpp:75-86);
+// expanding both to leaves lets the near-field kernels treat every entry as a leaf pair.
+__global__ void k_p2p_count(int64_t nrows, const int32_t* __restrict__ rtc, const int32_t* __restrict__ rsc,
+                            const int32_t* __restrict__ boff, const int32_t* __restrict__ bnum,
+                            const int32_t* __restrict__ rank_incl, int64_t* __restrict__ cnt) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
-    const int t = rtc[r];
-    const int lo = rank_incl[boff[t]] - 1, hi = rank_incl[boff[t] + bnum[t] - 1] - 1;
-    cnt[r] = hi - lo + 1;
+    const int t = rtc[r], sc = rsc[r];
+    const int64_t nt = rank_incl[boff[t] + bnum[t] - 1] - rank_incl[boff[t]] + 1;
+    const int64_t ns = rank_incl[boff[sc] + bnum[sc] - 1] - rank_incl[boff[sc]] + 1;
+    cnt[r] = nt * ns;
 }
 
 __global__ void k_p2p_expand(int64_t nrows, const int32_t* __restrict__ rtc, const int32_t* __restrict__ rsc,
                              const int32_t* __restrict__ roi, const int32_t* __restrict__ boff,
-                             const int32_t* __restrict__ rank_incl, const int64_t* __restrict__ cnt,
+                             const int32_t* __restrict__ bnum, const int32_t* __restrict__ rank_incl,
+                             const int32_t* __restrict__ leaf_cell, const int64_t* __restrict__ cnt,
                              const int64_t* __restrict__ incl, int32_t* __restrict__ ekey, int32_t* __restrict__ eval) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
+    const int sc = rsc[r];
     const int lo = rank_incl[boff[rtc[r]]] - 1;
+    const int slo = rank_incl[boff[sc]] - 1, ns = rank_incl[boff[sc] + bnum[sc] - 1] - slo;
     const int64_t c = cnt[r], b = incl[r] - c;
-    const int32_t v = (rsc[r] << 5) | roi[r];
-    for (int64_t k = 0; k < c; k++) { ekey[b + k] = lo + (int)k; eval[b + k] = v; }
+    for (int64_t k = 0; k < c; k++) {
+        const int tl = (int)(k / ns), sl = (int)(k % ns);
+        ekey[b + k] = lo + tl;
+        eval[b + k] = (leaf_cell[slo + sl] << 5) | roi[r];
+    }
 }
 
 __global__ void k_csr_offsets(int nleaf, int64_t ne, const int32_t* __restrict__ keys, int64_t* __restrict__ off) {
@@ -689,17 +698,18 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     CK(p2p_off.ensure((size_t)(nleaf + 1) * 8));
     if (n_rows > 0) {
         CK(cnt64.ensure((size_t)n_rows * 8)); CK(scan64.ensure((size_t)n_rows * 8));
-        k_p2p_count<<<blocks(n_rows), 256, 0, st>>>(n_rows, rows_tc.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
-                                                    rank.as<int32_t>(), cnt64.as<int64_t>());
+        k_p2p_count<<<blocks(n_rows), 256, 0, st>>>(n_rows, rows_tc.as<int32_t>(), rows_sc.as<int32_t>(),
+                                                    boff.as<int32_t>(), bnum.as<int32_t>(), rank.as<int32_t>(),
+                                                    cnt64.as<int64_t>());
         CK(hipGetLastError());
         CK(scan_incl(temp, cnt64.as<int64_t>(), scan64.as<int64_t>(), n_rows, st));
         CK(read_back(scan64.as<int64_t>() + (n_rows - 1), &n_entries, 1, st));
         CK(ekey.ensure((size_t)n_entries * 4)); CK(eval.ensure((size_t)n_entries * 4));
         CK(ekey_s.ensure((size_t)n_entries * 4)); CK(p2p_ent.ensure((size_t)n_entries * 4));
         k_p2p_expand<<<blocks(n_rows), 256, 0, st>>>(n_rows, rows_tc.as<int32_t>(), rows_sc.as<int32_t>(),
-                                                     rows_oi.as<int32_t>(), boff.as<int32_t>(), rank.as<int32_t>(),
-                                                     cnt64.as<int64_t>(), scan64.as<int64_t>(), ekey.as<int32_t>(),
-                                                     eval.as<int32_t>());
+                                                     rows_oi.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
+                                                     rank.as<int32_t>(), leaf_cell.as<int32_t>(), cnt64.as<int64_t>(),
+                                                     scan64.as<int64_t>(), ekey.as<int32_t>(), eval.as<int32_t>());
         CK(hipGetLastError());
         CK(sort_pairs(temp, ekey.as<uint32_t>(), ekey_s.as<uint32_t>(), eval.as<int32_t>(), p2p_ent.as<int32_t>(),
                       n_entries, bits_for(nleaf), st));

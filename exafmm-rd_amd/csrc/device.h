@@ -1,8 +1,10 @@
-// Device-side views and kernel launchers of the periodic FMM (float64 path).
+// Device-side views and kernel launchers of the periodic FMM (float64 and mixed-precision paths).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "dbuf.h"
 
 namespace fmm3dpp {
 
@@ -48,25 +50,62 @@ hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st);
 // Y[out[c]][r] = sum_{z < nz} part[z][c][r] for out[c] >= 0 (fixed order: deterministic M2M child sum)
 hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
                                hipStream_t st);
-hipError_t launch_m2l_up(int P, int S, int nsrc, const int32_t* src_child, const int32_t* lattice,
-                         const double* twiddle, const double* qeq, double* Qk, hipStream_t st);
-hipError_t launch_hadamard8(int F, int ntar, int nsrc, const int32_t* slot_src_T, const double* gk8, const double* Qk,
-                            double* Pk, hipStream_t st);
-hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src, int ntar,
-                             int nsrc, const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);
-hipError_t launch_m2l_down(int P, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
-                           const int32_t* lattice, const double* twiddle, const double* Pk, double* pck,
+// M2L (f32 selects the float32 complex layout of Qk/Pk and gk8; the transforms read q_equiv and
+// accumulate p_check in float64 either way).  Qk/Pk are [parent][f][8 children] complex;
+// inv_lat maps the P^3 corner of the convolution grid ((k * P + j) * P + i) to a surface point or -1.
+hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
+                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st);
+// targets t < ntar, slot_src_T rows of stride ntab
+hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
+                            const void* Qk, void* Pk, hipStream_t st);
+// Hadamard with per-chunk source staging in LDS.  Chunks are hadamard_chunk_targets(f32) consecutive
+// targets; u_src[chunk * ucap + u] (ucap >= 26 * chunk targets) lists a chunk's distinct sources,
+// u_cnt[chunk] their number, t_slot[p * ntar + t] the position of (t, p)'s source or -1.  umax =
+// max u_cnt; the Hadamard needs hadamard_lds_bytes(f32, umax) <= 160 KiB of LDS.
+int hadamard_chunk_targets(bool f32);
+size_t hadamard_lds_bytes(bool f32, int umax);
+hipError_t launch_m2l_union(bool f32, int ntar, int ntab, const int32_t* slot_src_T, int ucap, int32_t* u_src,
+                            int32_t* u_cnt, int16_t* t_slot, hipStream_t st);
+hipError_t launch_hadamard8_lds(bool f32, int F, int ntar, int ucap, int umax, const int32_t* u_src,
+                                const int32_t* u_cnt, const int16_t* t_slot, const void* gk8, const void* Qk,
+                                void* Pk, hipStream_t st);
+// float64 1x27 image lattice; img_src_T rows have stride ntab, Qk/Pk hold the parents from index base on
+hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,
+                             const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);
+hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
+                           const int32_t* inv_lat, const double* twiddle, const void* Pk, double* pck,
                            hipStream_t st);
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
                            hipStream_t st);
-// list_ent[e] = source cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle)
+// list_ent[e] = source leaf cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle)
 hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
                       const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
                       hipStream_t st);
-hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 3 * nblocks */, int nblocks, hipStream_t st);
-hipError_t launch_finalize(int n, const int32_t* perm, const double* q, const double* p, const double* gx,
+// float32 body records (x, y, z relative to the leaf centre, q) and the float32 P2P over them
+hipError_t launch_pack_rel(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, float4* rel,
+                           hipStream_t st);
+hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, const float4* rel,
+                          const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
+                          double* gy, double* gz, hipStream_t st);
+// partial[5 * block + (0..4)] = sum q x, sum q y, sum q z, sum q |x|^2, sum q
+hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 5 * nblocks */, int nblocks, hipStream_t st);
+// dL/dq and dL/dx from w = dL/dphi, grad phi, and (psi, grad psi) = the evaluation with charges w
+hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const double* partial_w, int nblocks,
+                                  double c, double out_scale, const double* qin, const double* win,
+                                  const double* grad_phi, const double* psi, const double* grad_psi, double* dq,
+                                  double* dx, hipStream_t st);
+// mode: FMM3DPP_DIPOLE_* (0 off, 1 reference, 2 infinite-lattice gauge)
+hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
                            const double* gy, const double* gz, const double* partial, int nblocks, int mode,
                            double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
+
+// Tin-foil Ewald sum at ntar target bodies (device arrays), float64; K = max |k_i| <= 23.
+struct EwaldScratch {
+    DBuf sk_part, val, part;
+};
+hipError_t ewald_device(int64_t n, const double* x, const double* y, const double* z, const double* q, int ntar,
+                        const int32_t* tar, double cycle, int K, double alpha, double cutoff, double* phi,
+                        double* grad, EwaldScratch& ws, hipStream_t st);
 
 }  // namespace fmm3dpp

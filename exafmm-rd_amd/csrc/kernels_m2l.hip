@@ -1,0 +1,673 @@
+// FFT-based M2L of the periodic KIFMM for CDNA4 (gfx950): up transform, 8x8 Hadamard product on
+// the matrix cores, 1x27 image-lattice Hadamard, down transform.
+//
+// Data layout in HBM (T = double for the float64 path, float for the mixed path):
+//   Qk, Pk   [parent][f][8 children] complex T, i.e. one 16-real vector per (parent, f): the
+//            Hadamard reads and writes whole 128-byte (float64) or 64-byte (float32) vectors, and the
+//            up/down transforms stream each parent's spectrum contiguously.  The Hadamard runs one
+//            frequency per workgroup with consecutive frequencies on one XCD, so the vectors of one
+//            frequency are re-read from that XCD's L2.
+//   inv_lat  [k][j][i] -> surface point index s on the P^3 corner of the convolution grid, or -1
+//   gk8      [26 patterns][f][is * 8 + it] complex T
+// Each kernel cites the reference loop it replaces (paths under 3dpp/).
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace fmm3dpp {
+
+namespace {
+
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+template <typename T> using cplx = typename Cplx<T>::type;
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mkc(T re, T im) {
+    cplx<T> v;
+    v.x = re;
+    v.y = im;
+    return v;
+}
+
+// M2L up (src/kernel.cpp:1286-1328): scatter each child's S equivalent charges on the P^3 corner of
+// the (2P-1)^3 grid and r2c-transform it as three pruned DFT passes.  One workgroup per source
+// parent and group of SG child slots.  The x and y passes of FC half-spectrum columns fi run fused
+// per (slot, k, fi) thread (P inputs -> P x-outputs in registers -> N y-outputs into LDS); the z
+// pass reads P values per (slot, fj, fi) and writes its N outputs straight to Qk[src][f][slot], so
+// consecutive lanes store consecutive complex values.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_m2l_up(int S, int SG, int FC, const int32_t* __restrict__ src_child,
+                                                const int32_t* __restrict__ inv_lat, const double* __restrict__ twiddle,
+                                                const double* __restrict__ qeq, cplx<T>* __restrict__ Qk) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int N = 2 * P - 1, Nh = N / 2 + 1, F = N * N * Nh, P3 = P * P * P;
+    const int ngrp = 8 / SG;
+    const int src = blockIdx.x / ngrp, slot0 = (blockIdx.x % ngrp) * SG, tid = threadIdx.x;
+    cplx<T>* tw = reinterpret_cast<cplx<T>*>(lds_raw);          // N
+    cplx<T>* T2c = tw + N;                                       // [sl][k][fj][fil]  SG*P*N*FC
+    T* L = reinterpret_cast<T*>(T2c + SG * P * N * FC);          // [sl][k][j][i]     SG*P^3
+    for (int m = tid; m < N; m += 256) tw[m] = mkc<T>((T)twiddle[2 * m], (T)twiddle[2 * m + 1]);
+    for (int e = tid; e < SG * P3; e += 256) {
+        const int sl = e / P3, g = e - sl * P3;
+        const int c = src_child[src * 8 + slot0 + sl], sp = inv_lat[g];
+        L[e] = (c >= 0 && sp >= 0) ? (T)qeq[(size_t)c * S + sp] : (T)0;
+    }
+    for (int fi0 = 0; fi0 < Nh; fi0 += FC) {
+        const int fc = min(FC, Nh - fi0);
+        __syncthreads();
+        for (int it = tid; it < SG * P * fc; it += 256) {   // x then y
+            const int fil = it % fc, r = it / fc, k = r % P, sl = r / P, fi = fi0 + fil;
+            const T* Lr = L + sl * P3 + k * P * P;
+            cplx<T> t1[P];
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                T re = 0, im = 0;
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    const T v = Lr[j * P + i];
+                    const cplx<T> w = tw[(i * fi) % N];
+                    re += v * w.x;
+                    im += v * w.y;
+                }
+                t1[j] = mkc<T>(re, im);
+            }
+            cplx<T>* dst = T2c + (size_t)(sl * P + k) * N * fc + fil;
+#pragma unroll
+            for (int fj = 0; fj < N; fj++) {
+                T re = 0, im = 0;
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                    const cplx<T> w = tw[(j * fj) % N];
+                    re += t1[j].x * w.x - t1[j].y * w.y;
+                    im += t1[j].x * w.y + t1[j].y * w.x;
+                }
+                dst[fj * fc] = mkc<T>(re, im);
+            }
+        }
+        __syncthreads();
+        for (int it = tid; it < SG * fc * N; it += 256) {   // z, written as Qk[src][f][slot]
+            const int sl = it % SG, r = it / SG, fil = r % fc, fj = r / fc, fi = fi0 + fil;
+            const cplx<T>* sp = T2c + (size_t)(sl * P * N + fj) * fc + fil;
+            cplx<T> v[P];
+#pragma unroll
+            for (int k = 0; k < P; k++) v[k] = sp[k * N * fc];
+            cplx<T>* out = Qk + ((size_t)src * F + fj * Nh + fi) * 8 + slot0 + sl;
+#pragma unroll
+            for (int fk = 0; fk < N; fk++) {
+                T re = 0, im = 0;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const cplx<T> w = tw[(k * fk) % N];
+                    re += v[k].x * w.x - v[k].y * w.y;
+                    im += v[k].x * w.y + v[k].y * w.x;
+                }
+                out[(size_t)fk * N * Nh * 8] = mkc<T>(re, im);
+            }
+        }
+    }
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { using type = f64x4; };
+template <> struct Vec4<float> { using type = f32x4; };
+template <typename T> using vec4 = typename Vec4<T>::type;
+
+// D(16x16) += A(16x4) B(4x16) on the matrix cores.  Lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; output register i holds D[4 (l / 16) + i][l % 16] for float32 and
+// D[l / 16 + 4 i][l % 16] for float64 (both measured on gfx950).  d_row<T>(g, i) is that row.
+template <typename T> __device__ __forceinline__ int d_row(int g, int i);
+template <> __device__ __forceinline__ int d_row<float>(int g, int i) { return 4 * g + i; }
+template <> __device__ __forceinline__ int d_row<double>(int g, int i) { return g + 4 * i; }
+__device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Hadamard 8x8 (src/kernel.cpp:1012-1082, include/c2c.h:201-296) on the matrix cores.  Targets
+// t < ntar, source table rows slot_src_T[p * ntab + t].  For one frequency f:  Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
+// Each complex 8x8 G_p acts as the real 16x16 matrix M on interleaved (re, im) vectors:
+//   M[2it][2is] = M[2it+1][2is+1] = Re G[is][it],  M[2it][2is+1] = -Im G,  M[2it+1][2is] = Im G.
+// A wave multiplies M (A operand, rows = output reals) by 16 targets' source vectors (B operand,
+// columns = targets) with four K=4 steps; lane (n, kr) feeds reals 4kr..4kr+3 of target n's source
+// vector, so the K order is permuted (step kk, lane group kr -> real 4kr + kk) identically in A and
+// B.  The A rows are permuted too, so that output register i of lane (n, kr) is real 4kr + i of
+// target n whatever the accumulator layout of the type: A row d_row(g, i) holds output real 4g + i.  TT tiles of 16 targets per wave, 4 waves per workgroup, one frequency per workgroup; blocks
+// are remapped so that consecutive frequency slices run on one XCD.
+template <typename T, int TT>
+__global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int ntab, int nchunk, int total,
+                                                        const int32_t* __restrict__ slot_src_T,
+                                                        const T* __restrict__ gk8, const T* __restrict__ Qk,
+                                                        T* __restrict__ Pk) {
+    __shared__ T gs[26 * 128];
+    const int per = (total + 7) / 8;
+    const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
+    if (Lb >= total) return;                                    // uniform over the workgroup
+    const int f = Lb / nchunk, chunk = Lb % nchunk;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < 26 * 128; e += 256) {
+        const int p = e >> 7, m = e & 127;
+        gs[e] = gk8[((size_t)p * F + f) * 128 + m];
+    }
+    __syncthreads();
+    const int n = lane & 15, kr = lane >> 4;
+    // A row n carries output real o = 4g + i where d_row(g, i) = n; o = 2 it + (re 0 / im 1)
+    int o = n;
+    for (int g = 0; g < 4; g++)
+        for (int i = 0; i < 4; i++)
+            if (d_row<T>(g, i) == n) o = 4 * g + i;
+    const int itA = o >> 1, imA = o & 1;
+    const int tbase = (chunk * 4 + wave) * 16 * TT;
+    // Software pipeline over the 26 patterns: source indices two patterns ahead, source vectors one
+    // pattern ahead, all loads unconditional (clamped addresses, zeroed afterwards), so that the
+    // matrix cores never wait on a dependent chain of memory round trips.
+    int tcl[TT];
+    bool tin[TT];
+#pragma unroll
+    for (int j = 0; j < TT; j++) {
+        const int t = tbase + 16 * j + n;
+        tin[j] = t < ntar;
+        tcl[j] = tin[j] ? t : ntar - 1;
+    }
+    auto load_idx = [&](int p, int* sidx) {
+#pragma unroll
+        for (int j = 0; j < TT; j++) {
+            const int v = slot_src_T[(size_t)p * ntab + tcl[j]];
+            sidx[j] = tin[j] ? v : -1;
+        }
+    };
+    auto load_vec = [&](const int* sidx, vec4<T>* raw) {   // zeroed for sidx < 0 only at use
+#pragma unroll
+        for (int j = 0; j < TT; j++)
+            raw[j] = *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)(sidx[j] >= 0 ? sidx[j] : 0) * F + f) * 16 + 4 * kr);
+    };
+    vec4<T> acc[TT], braw[TT], nraw[TT];
+    int s0[TT], s1[TT], s2[TT];
+#pragma unroll
+    for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
+    load_idx(0, s0);
+    load_vec(s0, braw);
+    load_idx(1, s1);
+#pragma unroll   // the buffer rotation below becomes register renaming
+    for (int p = 0; p < 26; p++) {
+        // keep the stages in place: the scheduler would otherwise hoist the next pattern's address
+        // arithmetic, and with it a wait on the loads just issued, into this one
+        __builtin_amdgcn_sched_barrier(0);
+        if (p + 1 < 26) load_vec(s1, nraw);
+        if (p + 2 < 26) load_idx(p + 2, s2);
+        const T* gp = gs + p * 128;
+        const T g0r = gp[((2 * kr) * 8 + itA) * 2], g0i = gp[((2 * kr) * 8 + itA) * 2 + 1];
+        const T g1r = gp[((2 * kr + 1) * 8 + itA) * 2], g1i = gp[((2 * kr + 1) * 8 + itA) * 2 + 1];
+        const T a0 = imA ? g0i : g0r, a1 = imA ? g0r : -g0i, a2 = imA ? g1i : g1r, a3 = imA ? g1r : -g1i;
+        vec4<T> bv[TT];
+#pragma unroll
+        for (int j = 0; j < TT; j++) bv[j] = s0[j] >= 0 ? braw[j] : vec4<T>{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a0, bv[j][0], acc[j]);
+#pragma unroll
+        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a1, bv[j][1], acc[j]);
+#pragma unroll
+        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a2, bv[j][2], acc[j]);
+#pragma unroll
+        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a3, bv[j][3], acc[j]);
+#pragma unroll
+        for (int j = 0; j < TT; j++) {
+            braw[j] = nraw[j];
+            s0[j] = s1[j];
+            s1[j] = s2[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < TT; j++) {
+        const int t = tbase + 16 * j + n;
+        if (t < ntar) *reinterpret_cast<vec4<T>*>(Pk + ((size_t)t * F + f) * 16 + 4 * kr) = acc[j];
+    }
+}
+
+// Source staging for the Hadamard: per chunk of CT consecutive targets (Morton-contiguous, so a
+// compact block of cells), the sorted distinct sources of its 26 x CT (target, pattern) pairs and,
+// per pair, the source's position in that list (-1: none).  One workgroup per chunk: keys sorted
+// bitonically in LDS, compacted with a block scan, positions found by binary search.
+constexpr int kUnionKeys = 8192;   // >= 26 * CT rounded up to a power of two, CT <= 256
+__global__ __launch_bounds__(256) void k_m2l_union(int ntar, int ntab, int CT, int nkeys,
+                                                   const int32_t* __restrict__ slot_src_T, int ucap,
+                                                   int32_t* __restrict__ u_src, int32_t* __restrict__ u_cnt,
+                                                   int16_t* __restrict__ t_slot) {
+    __shared__ int32_t key[kUnionKeys];
+    __shared__ int32_t uniq[26 * 256];
+    __shared__ int32_t scan[256];
+    const int chunk = blockIdx.x, tid = threadIdx.x;
+    for (int e = tid; e < nkeys; e += 256) key[e] = INT32_MAX;
+    __syncthreads();
+    for (int e = tid; e < 26 * CT; e += 256) {
+        const int p = e / CT, t = chunk * CT + e % CT;
+        const int sidx = t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
+        key[e] = sidx >= 0 ? sidx : INT32_MAX;
+    }
+    __syncthreads();
+    for (int k = 2; k <= nkeys; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < nkeys; i += 256) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = key[i], b = key[l];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    const int per = nkeys / 256, base = tid * per;
+    int cnt = 0;
+    for (int i = base; i < base + per; i++)
+        cnt += key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1]);
+    scan[tid] = cnt;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {   // inclusive scan
+        const int v = tid >= d ? scan[tid - d] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    int o = scan[tid] - cnt;
+    for (int i = base; i < base + per; i++)
+        if (key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1])) {
+            uniq[o] = key[i];
+            u_src[(size_t)chunk * ucap + o] = key[i];
+            o++;
+        }
+    __syncthreads();
+    const int U = scan[255];
+    if (tid == 0) u_cnt[chunk] = U;
+    for (int e = tid; e < 26 * CT; e += 256) {
+        const int p = e / CT, t = chunk * CT + e % CT;
+        if (t >= ntar) continue;
+        const int sidx = slot_src_T[(size_t)p * ntab + t];
+        int slot = -1;
+        if (sidx >= 0) {
+            int lo = 0, hi = U;   // first uniq >= sidx
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (uniq[mid] < sidx) lo = mid + 1;
+                else hi = mid;
+            }
+            slot = lo;
+        }
+        t_slot[(size_t)p * ntar + t] = (int16_t)slot;
+    }
+}
+
+// Hadamard 8x8 with the chunk's source vectors of frequency f staged in LDS (each read once from
+// HBM/L2 per workgroup instead of once per (target, pattern) pair).  Same MFMA scheme as
+// k_hadamard8_mfma; CT = 64 TT targets per workgroup (4 waves x TT tiles of 16).
+template <typename T, int TT>
+__global__ __launch_bounds__(256) void k_hadamard8_lds(int F, int ntar, int nchunk, int total, int ucap,
+                                                       const int32_t* __restrict__ u_src,
+                                                       const int32_t* __restrict__ u_cnt,
+                                                       const int16_t* __restrict__ t_slot,
+                                                       const T* __restrict__ gk8, const T* __restrict__ Qk,
+                                                       T* __restrict__ Pk) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* gs = reinterpret_cast<T*>(lds_raw);   // [26][128]
+    T* qs = gs + 26 * 128;                   // [U][16]
+    const int per = (total + 7) / 8;
+    const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
+    if (Lb >= total) return;                                    // uniform over the workgroup
+    const int f = Lb / nchunk, chunk = Lb % nchunk;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < 26 * 128; e += 256) {
+        const int p = e >> 7, m = e & 127;
+        gs[e] = gk8[((size_t)p * F + f) * 128 + m];
+    }
+    const int U = u_cnt[chunk];
+    const int32_t* us = u_src + (size_t)chunk * ucap;
+    for (int e = tid; e < 4 * U; e += 256) {
+        const int u = e >> 2, part = e & 3;
+        *reinterpret_cast<vec4<T>*>(qs + u * 16 + 4 * part) =
+            *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)us[u] * F + f) * 16 + 4 * part);
+    }
+    __syncthreads();
+    const int n = lane & 15, kr = lane >> 4;
+    int o = n;
+    for (int g = 0; g < 4; g++)
+        for (int i = 0; i < 4; i++)
+            if (d_row<T>(g, i) == n) o = 4 * g + i;
+    const int itA = o >> 1, imA = o & 1;
+    const int tbase = (chunk * 4 + wave) * 16 * TT;
+    vec4<T> acc[TT];
+#pragma unroll
+    for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
+    for (int p = 0; p < 26; p++) {
+        const T* gp = gs + p * 128;
+        const T g0r = gp[((2 * kr) * 8 + itA) * 2], g0i = gp[((2 * kr) * 8 + itA) * 2 + 1];
+        const T g1r = gp[((2 * kr + 1) * 8 + itA) * 2], g1i = gp[((2 * kr + 1) * 8 + itA) * 2 + 1];
+        const T a0 = imA ? g0i : g0r, a1 = imA ? g0r : -g0i, a2 = imA ? g1i : g1r, a3 = imA ? g1r : -g1i;
+        vec4<T> bv[TT];
+#pragma unroll
+        for (int j = 0; j < TT; j++) {
+            const int t = tbase + 16 * j + n;
+            const int sl = t < ntar ? t_slot[(size_t)p * ntar + t] : -1;
+            if (sl >= 0)
+                bv[j] = *reinterpret_cast<const vec4<T>*>(qs + sl * 16 + 4 * kr);
+            else
+                bv[j] = vec4<T>{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < TT; j++) {
+            acc[j] = mfma16x16x4(a0, bv[j][0], acc[j]);
+            acc[j] = mfma16x16x4(a1, bv[j][1], acc[j]);
+            acc[j] = mfma16x16x4(a2, bv[j][2], acc[j]);
+            acc[j] = mfma16x16x4(a3, bv[j][3], acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < TT; j++) {
+        const int t = tbase + 16 * j + n;
+        if (t < ntar) *reinterpret_cast<vec4<T>*>(Pk + ((size_t)t * F + f) * 16 + 4 * kr) = acc[j];
+    }
+}
+
+// Hadamard 1x27 (src/kernel.cpp:1084-1138): image-lattice M2L for the image parents c(-1..-(i-1)),
+// always in float64.  Every one of the 27 lattice children of a neighbouring image is the same
+// single child, so the 27 kernels of a pattern are pre-summed; only child slot 0 exists.
+// Target/source indices are global (row stride ntab of img_src_T), minus `base` in Qk/Pk.  All 8
+// slots of a target are written (slots 1..7 zero).
+__global__ __launch_bounds__(256) void k_hadamard27(int F, int ntab, int base,
+                                                    const int32_t* __restrict__ img_tar,
+                                                    const int32_t* __restrict__ img_src_T, const double* __restrict__ g27,
+                                                    const double2* __restrict__ Qk, double2* __restrict__ Pk) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int t = img_tar[blockIdx.y];
+    double re = 0, im = 0;
+    for (int p = 0; p < 26; p++) {
+        const int s = img_src_T[(size_t)p * ntab + t];
+        if (s < 0) continue;
+        double2 q = Qk[((size_t)(s - base) * F + f) * 8];
+        double gr = g27[((size_t)p * F + f) * 2], gi = g27[((size_t)p * F + f) * 2 + 1];
+        re += gr * q.x - gi * q.y;
+        im += gr * q.y + gi * q.x;
+    }
+    double2* o = Pk + ((size_t)(t - base) * F + f) * 8;
+    o[0] = make_double2(re, im);
+    for (int k = 1; k < 8; k++) o[k] = make_double2(0, 0);
+}
+
+// M2L down (src/kernel.cpp:1343-1385): c2r of each child's Pk evaluated only on the P^3 block that
+// holds the shifted check surface, then gather, scale (2 or 3)^(depth+1) / N^3 and accumulate into
+// p_check (float64).  One workgroup per target parent and group of SG child slots; the half
+// spectrum is taken FC columns fi at a time: z pass (N values of Pk[t][f][slot] -> P outputs),
+// y pass (N -> P), and the Hermitian x pass accumulates into per-thread registers, so only the two
+// column chunks live in LDS.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_m2l_down(int S, int SG, int FC, const int32_t* __restrict__ tar_child,
+                                                  const double* __restrict__ tar_scale,
+                                                  const int32_t* __restrict__ inv_lat,
+                                                  const double* __restrict__ twiddle, const cplx<T>* __restrict__ Pk,
+                                                  double* __restrict__ pck) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int N = 2 * P - 1, Nh = N / 2 + 1, N3 = N * N * N, F = N * N * Nh;
+    constexpr int XI = (8 * P * P + 255) / 256;                  // x-pass items per thread (SG <= 8)
+    const int ngrp = 8 / SG;
+    const int t = blockIdx.x / ngrp, slot0 = (blockIdx.x % ngrp) * SG, tid = threadIdx.x;
+    cplx<T>* tw = reinterpret_cast<cplx<T>*>(lds_raw);          // N
+    cplx<T>* T1c = tw + N;                                       // [sl][kk][fj][fil]  SG*P*N*FC
+    cplx<T>* T2c = T1c + SG * P * N * FC;                        // [sl][kk][jj][fil]  SG*P*P*FC
+    for (int m = tid; m < N; m += 256) tw[m] = mkc<T>((T)twiddle[2 * m], (T)twiddle[2 * m + 1]);
+    T acc[XI][P];
+#pragma unroll
+    for (int m = 0; m < XI; m++)
+#pragma unroll
+        for (int ii = 0; ii < P; ii++) acc[m][ii] = 0;
+    const cplx<T>* pk = Pk + (size_t)t * F * 8 + slot0;
+    for (int fi0 = 0; fi0 < Nh; fi0 += FC) {
+        const int fc = min(FC, Nh - fi0);
+        __syncthreads();
+        for (int it = tid; it < SG * fc * N; it += 256) {   // z: k = P-1+kk from all N fk
+            const int sl = it % SG, r = it / SG, fil = r % fc, fj = r / fc, fi = fi0 + fil;
+            T re[P], im[P];
+#pragma unroll
+            for (int kk = 0; kk < P; kk++) re[kk] = im[kk] = 0;
+#pragma unroll
+            for (int fk = 0; fk < N; fk++) {
+                const cplx<T> v = pk[(size_t)((fk * N + fj) * Nh + fi) * 8 + sl];
+#pragma unroll
+                for (int kk = 0; kk < P; kk++) {
+                    const cplx<T> w = tw[((P - 1 + kk) * fk) % N];   // conj(w): exp(+2 pi i k fk / N)
+                    re[kk] += v.x * w.x + v.y * w.y;
+                    im[kk] += v.y * w.x - v.x * w.y;
+                }
+            }
+            cplx<T>* dst = T1c + (size_t)(sl * P * N + fj) * fc + fil;
+#pragma unroll
+            for (int kk = 0; kk < P; kk++) dst[kk * N * fc] = mkc<T>(re[kk], im[kk]);
+        }
+        __syncthreads();
+        for (int it = tid; it < SG * P * fc; it += 256) {   // y: j = P-1+jj
+            const int fil = it % fc, r = it / fc;             // r = sl * P + kk
+            const cplx<T>* s1 = T1c + (size_t)r * N * fc + fil;
+            cplx<T> v[N];
+#pragma unroll
+            for (int fj = 0; fj < N; fj++) v[fj] = s1[fj * fc];
+            cplx<T>* dst = T2c + (size_t)r * P * fc + fil;
+#pragma unroll
+            for (int jj = 0; jj < P; jj++) {
+                T re = 0, im = 0;
+#pragma unroll
+                for (int fj = 0; fj < N; fj++) {
+                    const cplx<T> w = tw[((P - 1 + jj) * fj) % N];
+                    re += v[fj].x * w.x + v[fj].y * w.y;
+                    im += v[fj].y * w.x - v[fj].x * w.y;
+                }
+                dst[jj * fc] = mkc<T>(re, im);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < XI; m++) {                        // x: Hermitian half spectrum -> real
+            const int it = tid + 256 * m;                     // it = (sl * P + kk) * P + jj
+            if (it >= SG * P * P) continue;
+            const cplx<T>* s2 = T2c + (size_t)it * fc;
+            for (int fil = 0; fil < fc; fil++) {
+                const int fi = fi0 + fil;
+                const cplx<T> v = s2[fil];
+#pragma unroll
+                for (int ii = 0; ii < P; ii++) {
+                    if (fi == 0) {
+                        acc[m][ii] += v.x;
+                    } else {
+                        const cplx<T> w = tw[((P - 1 + ii) * fi) % N];
+                        acc[m][ii] += 2 * (v.x * w.x + v.y * w.y);
+                    }
+                }
+            }
+        }
+    }
+    const double scale = tar_scale[t] / N3;
+#pragma unroll
+    for (int m = 0; m < XI; m++) {
+        const int it = tid + 256 * m;
+        if (it >= SG * P * P) continue;
+        const int sl = it / (P * P), kj = it % (P * P);        // kj = kk * P + jj
+        const int c = tar_child[t * 8 + slot0 + sl];
+        if (c < 0) continue;
+#pragma unroll
+        for (int ii = 0; ii < P; ii++) {
+            const int sp = inv_lat[kj * P + ii];
+            if (sp >= 0) pck[(size_t)c * S + sp] += (double)acc[m][ii] * scale;
+        }
+    }
+}
+
+// LDS of the transforms for slot group SG and column chunk FC (element size esz).
+size_t m2l_up_lds(int P, int SG, int FC, size_t esz) {
+    const int N = 2 * P - 1;
+    return esz * (2 * N + 2 * SG * P * N * FC + SG * P * P * P);
+}
+size_t m2l_down_lds(int P, int SG, int FC, size_t esz) {
+    const int N = 2 * P - 1;
+    return esz * (2 * N + 2 * SG * P * N * FC + 2 * SG * P * P * FC);
+}
+// Largest column chunk (then slot group) that leaves room for two workgroups per CU; one
+// workgroup per CU if nothing smaller fits.
+void pick_tiles(size_t (*lds)(int, int, int, size_t), int P, size_t esz, int& SG, int& FC) {
+    const int Nh = P;   // N / 2 + 1 with N = 2P - 1
+    for (size_t budget : {(size_t)80 * 1024, (size_t)160 * 1024})
+        for (int sg = 8; sg >= 1; sg /= 2)
+            for (int fc = Nh; fc >= 1; fc--)
+                if (lds(P, sg, fc, esz) <= budget) { SG = sg; FC = fc; return; }
+    SG = 1;
+    FC = 1;
+}
+
+template <int P, typename T>
+hipError_t up_p(int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat, const double* twiddle,
+                const double* qeq, void* Qk, hipStream_t st) {
+    int SG, FC;
+    pick_tiles(m2l_up_lds, P, sizeof(T), SG, FC);
+    const size_t lds = m2l_up_lds(P, SG, FC, sizeof(T));
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_m2l_up<P, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    k_m2l_up<P, T><<<nsrc * (8 / SG), 256, lds, st>>>(S, SG, FC, src_child, inv_lat, twiddle, qeq,
+                                                       static_cast<cplx<T>*>(Qk));
+    return hipGetLastError();
+}
+
+template <int P, typename T>
+hipError_t down_p(int S, int ntar, const int32_t* tar_child, const double* tar_scale, const int32_t* inv_lat,
+                  const double* twiddle, const void* Pk, double* pck, hipStream_t st) {
+    int SG, FC;
+    pick_tiles(m2l_down_lds, P, sizeof(T), SG, FC);
+    const size_t lds = m2l_down_lds(P, SG, FC, sizeof(T));
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_m2l_down<P, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    k_m2l_down<P, T><<<ntar * (8 / SG), 256, lds, st>>>(S, SG, FC, tar_child, tar_scale, inv_lat, twiddle,
+                                                         static_cast<const cplx<T>*>(Pk), pck);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
+                       const void* Qk, void* Pk, hipStream_t st) {
+    constexpr int TT = 4;
+    const int nchunk = (ntar + 64 * TT - 1) / (64 * TT);
+    const int total = nchunk * F;
+    const int per = (total + 7) / 8;
+    k_hadamard8_mfma<T, TT><<<per * 8, 256, 0, st>>>(F, ntar, ntab, nchunk, total, slot_src_T,
+                                                     static_cast<const T*>(gk8), static_cast<const T*>(Qk),
+                                                     static_cast<T*>(Pk));
+    return hipGetLastError();
+}
+
+template <typename T, int TT>
+hipError_t hadamard8_lds_t(int F, int ntar, int ucap, int umax, const int32_t* u_src, const int32_t* u_cnt,
+                           const int16_t* t_slot, const void* gk8, const void* Qk, void* Pk, hipStream_t st) {
+    const int CT = 64 * TT;
+    const int nchunk = (ntar + CT - 1) / CT;
+    const int total = nchunk * F;
+    const int per = (total + 7) / 8;
+    const size_t lds = (size_t)(26 * 128 + 16 * (size_t)umax) * sizeof(T);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_hadamard8_lds<T, TT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    k_hadamard8_lds<T, TT><<<per * 8, 256, lds, st>>>(F, ntar, nchunk, total, ucap, u_src, u_cnt, t_slot,
+                                                      static_cast<const T*>(gk8), static_cast<const T*>(Qk),
+                                                      static_cast<T*>(Pk));
+    return hipGetLastError();
+}
+
+#define FMM3DPP_P_CASES(M) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12)
+
+}  // namespace
+
+hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
+                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st) {
+    if (nsrc <= 0) return hipSuccess;
+    switch (P) {
+#define UPCASE(p)                                                                                  \
+    case p:                                                                                        \
+        return f32 ? up_p<p, float>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st)             \
+                   : up_p<p, double>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st);
+        FMM3DPP_P_CASES(UPCASE)
+#undef UPCASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T,
+                            const void* gk8, const void* Qk, void* Pk, hipStream_t st) {
+    if (ntar <= 0) return hipSuccess;
+    return f32 ? hadamard8_t<float>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st)
+               : hadamard8_t<double>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st);
+}
+
+hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,
+                             const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st) {
+    if (nimg <= 0) return hipSuccess;
+    k_hadamard27<<<dim3((F + 255) / 256, nimg), 256, 0, st>>>(F, ntab, base, img_tar, img_src_T, gk27_sum,
+                                                              reinterpret_cast<const double2*>(Qk),
+                                                              reinterpret_cast<double2*>(Pk));
+    return hipGetLastError();
+}
+
+hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
+                           const int32_t* inv_lat, const double* twiddle, const void* Pk, double* pck,
+                           hipStream_t st) {
+    if (ntar <= 0) return hipSuccess;
+    switch (P) {
+#define DNCASE(p)                                                                                          \
+    case p:                                                                                                \
+        return f32 ? down_p<p, float>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st)        \
+                   : down_p<p, double>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st);
+        FMM3DPP_P_CASES(DNCASE)
+#undef DNCASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int hadamard_chunk_targets(bool f32) { return f32 ? 256 : 128; }
+
+size_t hadamard_lds_bytes(bool f32, int umax) { return (size_t)(26 * 128 + 16 * (size_t)umax) * (f32 ? 4 : 8); }
+
+hipError_t launch_m2l_union(bool f32, int ntar, int ntab, const int32_t* slot_src_T, int ucap, int32_t* u_src,
+                            int32_t* u_cnt, int16_t* t_slot, hipStream_t st) {
+    if (ntar <= 0) return hipSuccess;
+    const int CT = hadamard_chunk_targets(f32);
+    int nkeys = 256;
+    while (nkeys < 26 * CT) nkeys <<= 1;
+    if (nkeys > kUnionKeys || ucap < 26 * CT) return hipErrorInvalidValue;
+    k_m2l_union<<<(ntar + CT - 1) / CT, 256, 0, st>>>(ntar, ntab, CT, nkeys, slot_src_T, ucap, u_src, u_cnt, t_slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_hadamard8_lds(bool f32, int F, int ntar, int ucap, int umax, const int32_t* u_src,
+                                const int32_t* u_cnt, const int16_t* t_slot, const void* gk8, const void* Qk,
+                                void* Pk, hipStream_t st) {
+    if (ntar <= 0) return hipSuccess;
+    if (hadamard_lds_bytes(f32, umax) > 160 * 1024) return hipErrorInvalidValue;
+    return f32 ? hadamard8_lds_t<float, 4>(F, ntar, ucap, umax, u_src, u_cnt, t_slot, gk8, Qk, Pk, st)
+               : hadamard8_lds_t<double, 2>(F, ntar, ucap, umax, u_src, u_cnt, t_slot, gk8, Qk, Pk, st);
+}
+
+}  // namespace fmm3dpp

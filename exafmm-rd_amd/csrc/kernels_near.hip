@@ -1,0 +1,325 @@
+// Particle-level kernels of the periodic KIFMM for CDNA4 (gfx950): P2P near field (float64 and
+// float32 forms), P2M check potentials and L2P evaluation.
+//
+// All three are pair sums of 1/r over (target, source) point pairs inside one leaf's wave:
+//   lanes = W target slots x G source groups, W = the smallest of 8/16/32/64 that holds the leaf's
+//   bodies, G = 64 / W.  Group g takes every G-th source, the G partial sums of a target are
+//   folded with cross-lane shuffles.  Uniform trees hold about ncrit/2 bodies per leaf, so a lane
+//   per target alone would leave half of every wave idle.
+// 1/r uses the hardware reciprocal square root (v_rsq_f64 / v_rsq_f32).  Measured on gfx950 over
+// r^2 in [1e-12, 1e2]: v_rsq_f64 is within 5.2e-8 relative, one Newton step brings it to 4.1e-15
+// (the float64 form used here, far inside the float64 path's 1e-12 parity bound), v_rsq_f32 to
+// 1.1e-7.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace fmm3dpp {
+
+namespace {
+
+// 1/sqrt(r2) for r2 > 0, 0 for r2 == 0 (the reference's r > 0 mask, src/kernel.cpp:283-290)
+__device__ __forceinline__ double inv_sqrt(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    const double t = r2 * y * y;
+    y = y * fma(-0.5, t, 1.5);
+    return r2 > 0 ? y : 0.0;
+}
+
+__device__ __forceinline__ float inv_sqrt(float r2) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    return r2 > 0 ? y : 0.0f;
+}
+
+// log2 of the target slots W of a leaf with n bodies
+__device__ __forceinline__ int slot_bits(int n) { return n <= 8 ? 3 : n <= 16 ? 4 : n <= 32 ? 5 : 6; }
+
+template <typename T>
+__device__ __forceinline__ T fold_groups(T v, int wbits) {
+    for (int m = 1 << wbits; m < 64; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+// P2P, float64 (src/kernel.cpp:258-311 p2p_1toN_256 over the leaf's P2P list, fmm.cpp:259-294):
+// phi_t += q_s / r, grad_t -= q_s (x_t - x_s) / r^3 with x_s shifted by the list entry's periodic
+// offset.  One wave per target leaf.
+__global__ __launch_bounds__(64) void k_p2p_f64(const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
+                                                const int64_t* __restrict__ list_off,
+                                                const int32_t* __restrict__ list_ent, double cycle,
+                                                double* __restrict__ p, double* __restrict__ gx,
+                                                double* __restrict__ gy, double* __restrict__ gz) {
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    const int64_t e0 = list_off[li], e1 = list_off[li + 1];
+    const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
+    const int g = lane >> wb, tl = lane & (W - 1);
+    for (int t0 = 0; t0 < bnum; t0 += W) {
+        const int i = t0 + tl;
+        const bool active = i < bnum;
+        const int bi = boff + (active ? i : 0);
+        const double xt = b.x[bi], yt = b.y[bi], zt = b.z[bi];
+        double ap = 0, ax = 0, ay = 0, az = 0;
+        for (int64_t e = e0; e < e1; e++) {
+            const int ent = list_ent[e], sc = ent >> 5, oi = ent & 31;   // source cell, periodic offset index
+            const double xo = xt - (oi % 3 - 1) * cycle, yo = yt - ((oi / 3) % 3 - 1) * cycle,
+                         zo = zt - (oi / 9 - 1) * cycle;
+            const int sb = cells.boff[sc], se = sb + cells.bnum[sc];
+            for (int j = sb + g; j < se; j += G) {
+                const double dx = xo - b.x[j], dy = yo - b.y[j], dz = zo - b.z[j];
+                const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+                double qi = b.q[j] * ir;
+                ap += qi;
+                qi *= ir * ir;
+                ax += qi * dx;
+                ay += qi * dy;
+                az += qi * dz;
+            }
+        }
+        ap = fold_groups(ap, wb);
+        ax = fold_groups(ax, wb);
+        ay = fold_groups(ay, wb);
+        az = fold_groups(az, wb);
+        if (active && g == 0) {
+            p[bi] += ap;
+            gx[bi] -= ax;
+            gy[bi] -= ay;
+            gz[bi] -= az;
+        }
+    }
+}
+
+// float32 body records for the mixed-precision P2P: (x, y, z) relative to the body's own leaf
+// centre and q, so that every float32 coordinate is of the order of the leaf size.
+__global__ __launch_bounds__(64) void k_pack_rel(const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
+                                                 float4* __restrict__ rel) {
+    const int c = leaf_cell[blockIdx.x];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c];
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    for (int i = threadIdx.x; i < bnum; i += 64) {
+        const int j = boff + i;
+        rel[j] = make_float4((float)(b.x[j] - cx), (float)(b.y[j] - cy), (float)(b.z[j] - cz), (float)b.q[j]);
+    }
+}
+
+// P2P, float32 with two-level summation: the pairs of one (target, source leaf) are summed in
+// float32, each source leaf's partial is added to float64 accumulators.  The source leaf's
+// displacement (its centre + periodic offset - the target leaf centre) is formed in float64 once
+// per list entry, so no float32 coordinate carries an absolute position.
+__global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf_cell, CellsDev cells,
+                                                const float4* __restrict__ rel, const int64_t* __restrict__ list_off,
+                                                const int32_t* __restrict__ list_ent, double cycle,
+                                                double* __restrict__ p, double* __restrict__ gx,
+                                                double* __restrict__ gy, double* __restrict__ gz) {
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c];
+    const int64_t e0 = list_off[li], e1 = list_off[li + 1];
+    const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
+    const int g = lane >> wb, tl = lane & (W - 1);
+    for (int t0 = 0; t0 < bnum; t0 += W) {
+        const int i = t0 + tl;
+        const bool active = i < bnum;
+        const int bi = boff + (active ? i : 0);
+        const float4 tv = rel[bi];
+        double ap = 0, ax = 0, ay = 0, az = 0;
+        for (int64_t e = e0; e < e1; e++) {
+            const int ent = list_ent[e], sc = ent >> 5, oi = ent & 31;
+            const float xo = tv.x - (float)(cells.x[sc] + (oi % 3 - 1) * cycle - cx);
+            const float yo = tv.y - (float)(cells.y[sc] + ((oi / 3) % 3 - 1) * cycle - cy);
+            const float zo = tv.z - (float)(cells.z[sc] + (oi / 9 - 1) * cycle - cz);
+            const int sb = cells.boff[sc], se = sb + cells.bnum[sc];
+            float fp = 0, fx = 0, fy = 0, fz = 0;
+            for (int j = sb + g; j < se; j += G) {
+                const float4 sv = rel[j];
+                const float dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
+                const float ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+                float qi = sv.w * ir;
+                fp += qi;
+                qi *= ir * ir;
+                fx += qi * dx;
+                fy += qi * dy;
+                fz += qi * dz;
+            }
+            ap += fp;
+            ax += fx;
+            ay += fy;
+            az += fz;
+        }
+        ap = fold_groups(ap, wb);
+        ax = fold_groups(ax, wb);
+        ay = fold_groups(ay, wb);
+        az = fold_groups(az, wb);
+        if (active && g == 0) {
+            p[bi] += ap;
+            gx[bi] -= ax;
+            gy[bi] -= ay;
+            gz[bi] -= az;
+        }
+    }
+}
+
+// P2M source -> check-surface potentials (src/kernel.cpp:332-368): one wave per leaf, KP check
+// points per lane, the leaf's bodies staged through LDS 64 at a time.
+template <int KP>
+__global__ __launch_bounds__(64) void k_p2m_check(int S, const double* __restrict__ usurf,
+                                                  const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
+                                                  double* __restrict__ pc) {
+    __shared__ double sx[64], sy[64], sz[64], sq[64];
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    double px[KP], py[KP], pz[KP], acc[KP];
+#pragma unroll
+    for (int m = 0; m < KP; m++) {
+        const int s = lane + 64 * m;
+        acc[m] = 0;
+        const int sr = s < S ? s : 0;   // slots past S evaluate point 0 and are not written
+        px[m] = usurf[3 * sr] * rr + cx;
+        py[m] = usurf[3 * sr + 1] * rr + cy;
+        pz[m] = usurf[3 * sr + 2] * rr + cz;
+    }
+    for (int b0 = 0; b0 < bnum; b0 += 64) {
+        const int nb = min(64, bnum - b0);
+        __syncthreads();
+        if (lane < nb) {
+            sx[lane] = b.x[boff + b0 + lane];
+            sy[lane] = b.y[boff + b0 + lane];
+            sz[lane] = b.z[boff + b0 + lane];
+            sq[lane] = b.q[boff + b0 + lane];
+        }
+        __syncthreads();
+        for (int j = 0; j < nb; j++) {
+            const double xj = sx[j], yj = sy[j], zj = sz[j], qj = sq[j];
+#pragma unroll
+            for (int m = 0; m < KP; m++) {
+                const double dx = px[m] - xj, dy = py[m] - yj, dz = pz[m] - zj;
+                acc[m] += qj * inv_sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < KP; m++) {
+        const int s = lane + 64 * m;
+        if (s < S) pc[(size_t)li * S + s] = acc[m];
+    }
+}
+
+// L2P evaluation (src/kernel.cpp:1471-1510): potential and gradient at the leaf's bodies from the
+// S downward equivalent charges on the 2.95 r surface.  One wave per leaf, equivalent points in
+// LDS, lanes split as in P2P.  Writes p and grad (the first contribution of the evaluation).
+__global__ __launch_bounds__(64) void k_l2p_eval(int S, const double* __restrict__ usurf,
+                                                 const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
+                                                 const double* __restrict__ qdown, double* __restrict__ p,
+                                                 double* __restrict__ gx, double* __restrict__ gy,
+                                                 double* __restrict__ gz) {
+    extern __shared__ double lds[];
+    double* ex = lds;
+    double* ey = ex + S;
+    double* ez = ey + S;
+    double* eq = ez + S;
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
+    for (int s = lane; s < S; s += 64) {
+        ex[s] = usurf[3 * s] * rr + cx;
+        ey[s] = usurf[3 * s + 1] * rr + cy;
+        ez[s] = usurf[3 * s + 2] * rr + cz;
+        eq[s] = qdown[(size_t)li * S + s];
+    }
+    __syncthreads();
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
+    const int g = lane >> wb, tl = lane & (W - 1);
+    for (int t0 = 0; t0 < bnum; t0 += W) {
+        const int i = t0 + tl;
+        const bool active = i < bnum;
+        const int bi = boff + (active ? i : 0);
+        const double xt = b.x[bi], yt = b.y[bi], zt = b.z[bi];
+        double ap = 0, ax = 0, ay = 0, az = 0;
+        for (int s = g; s < S; s += G) {
+            const double dx = xt - ex[s], dy = yt - ey[s], dz = zt - ez[s];
+            const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+            double qi = eq[s] * ir;
+            ap += qi;
+            qi *= ir * ir;
+            ax += qi * dx;
+            ay += qi * dy;
+            az += qi * dz;
+        }
+        ap = fold_groups(ap, wb);
+        ax = fold_groups(ax, wb);
+        ay = fold_groups(ay, wb);
+        az = fold_groups(az, wb);
+        if (active && g == 0) {
+            p[bi] = ap;
+            gx[bi] = -ax;
+            gy[bi] = -ay;
+            gz[bi] = -az;
+        }
+    }
+}
+
+template <int KP>
+hipError_t p2m_kp(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b,
+                  double* pc, hipStream_t st) {
+    k_p2m_check<KP><<<nleaf, 64, 0, st>>>(S, usurf, leaf_cell, cells, b, pc);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
+                            BodiesDev b, double* pc, hipStream_t st) {
+    if (nleaf <= 0) return hipSuccess;
+    switch ((S + 63) / 64) {
+        case 1: return p2m_kp<1>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 2: return p2m_kp<2>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 3: return p2m_kp<3>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 4: return p2m_kp<4>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 5: return p2m_kp<5>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 6: return p2m_kp<6>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 7: return p2m_kp<7>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 8: return p2m_kp<8>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 9: return p2m_kp<9>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 10: return p2m_kp<10>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 11: return p2m_kp<11>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 12: return p2m_kp<12>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        default: return hipErrorInvalidValue;   // S <= 728 (P <= 12)
+    }
+}
+
+hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
+                           BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
+                           hipStream_t st) {
+    if (nleaf <= 0) return hipSuccess;
+    k_l2p_eval<<<nleaf, 64, 4 * S * sizeof(double), st>>>(S, usurf, leaf_cell, cells, b, qdown, p, gx, gy, gz);
+    return hipGetLastError();
+}
+
+hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
+                      const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
+                      hipStream_t st) {
+    if (nleaf <= 0) return hipSuccess;
+    k_p2p_f64<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_ent, cycle, p, gx, gy, gz);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_rel(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, float4* rel,
+                           hipStream_t st) {
+    if (nleaf <= 0) return hipSuccess;
+    k_pack_rel<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, rel);
+    return hipGetLastError();
+}
+
+hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, const float4* rel,
+                          const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
+                          double* gy, double* gz, hipStream_t st) {
+    if (nleaf <= 0) return hipSuccess;
+    k_p2p_f32<<<nleaf, 64, 0, st>>>(leaf_cell, cells, rel, list_off, list_ent, cycle, p, gx, gy, gz);
+    return hipGetLastError();
+}
+
+}  // namespace fmm3dpp

@@ -8,6 +8,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -43,7 +44,15 @@ struct fmm3dpp_plan {
 
     DBuf d_inx, d_iny, d_inz, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
     DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
-    DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_all, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk27, o_usurf, o_lattice, o_tw;
+    DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
+    DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
+    DBuf h_usrc, h_ucnt, h_tslot;     // Hadamard source staging (per chunk of targets)
+    int h_ucap = 0, h_umax = 0;
+    bool h_lds = false;
+    DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_all, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk8f, o_gk27, o_usurf, o_invlat,
+        o_tw;
+
+    bool mixed() const { return cfg.precision == FMM3DPP_PREC_MIXED; }
 
     CellsDev cells_dev() const {
         return {gs.cx.as<double>(), gs.cy.as<double>(), gs.cz.as<double>(), gs.cr.as<double>(), gs.depth.as<int32_t>(),
@@ -78,8 +87,8 @@ int validate(const fmm3dpp_config* c, std::string& err) {
     for (int k = 0; k < 3; k++)
         if (!std::isfinite(c->center[k])) { err = "center must be finite"; return FMM3DPP_ERR_ARG; }
     if (c->precision != FMM3DPP_PREC_FP64 && c->precision != FMM3DPP_PREC_MIXED) { err = "unknown precision"; return FMM3DPP_ERR_ARG; }
-    if (c->precision == FMM3DPP_PREC_MIXED) { err = "mixed precision is not available in this build"; return FMM3DPP_ERR_ARG; }
-    if (c->dipole_mode != FMM3DPP_DIPOLE_OFF && c->dipole_mode != FMM3DPP_DIPOLE_REFERENCE) { err = "unknown dipole mode"; return FMM3DPP_ERR_ARG; }
+    if (c->dipole_mode != FMM3DPP_DIPOLE_OFF && c->dipole_mode != FMM3DPP_DIPOLE_REFERENCE &&
+        c->dipole_mode != FMM3DPP_DIPOLE_PHYSICAL) { err = "unknown dipole mode"; return FMM3DPP_ERR_ARG; }
     return FMM3DPP_OK;
 }
 
@@ -99,9 +108,19 @@ int upload_operators(fmm3dpp_plan* plan) {
     HIPCK(upload(plan->o_m2m_img, o.images > 0 ? o.m2m_img_sum : zeros, s));
     HIPCK(upload(plan->o_l2l_img, o.images > 0 ? o.l2l_img : zeros, s));
     HIPCK(upload(plan->o_gk8, o.gk8, s));
+    if (plan->mixed()) {
+        std::vector<float> g32(o.gk8.begin(), o.gk8.end());
+        HIPCK(upload(plan->o_gk8f, g32, s));
+    }
     HIPCK(upload(plan->o_gk27, o.gk27_sum, s));
     HIPCK(upload(plan->o_usurf, o.unit_surf, s));
-    HIPCK(upload(plan->o_lattice, o.lattice, s));
+    {   // (k * P + j) * P + i of the P^3 grid corner -> surface point (surface.h:28-48), -1 inside
+        const int P = o.P;
+        std::vector<int32_t> inv((size_t)P * P * P, -1);
+        for (int sp = 0; sp < o.S; sp++)
+            inv[((size_t)o.lattice[3 * sp + 2] * P + o.lattice[3 * sp + 1]) * P + o.lattice[3 * sp]] = sp;
+        HIPCK(upload(plan->o_invlat, inv, s));
+    }
     HIPCK(upload(plan->o_tw, o.twiddle, s));
     HIPCK(hipStreamSynchronize(s));
     plan->ops_uploaded = true;
@@ -139,9 +158,39 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
     HIPCK(plan->d_qeq.ensure(nc * Sn * 8));
     HIPCK(hipMemsetAsync(plan->d_qeq.p, 0, nc * Sn * 8, s));
     HIPCK(plan->d_pck.ensure(nc * Sn * 8));
-    HIPCK(plan->d_Qk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
-    HIPCK(plan->d_Pk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
-    HIPCK(plan->d_partial.ensure(256 * 3 * 8));
+    if (plan->mixed()) {   // float32 spectra of the tree parents, float64 spectra of the image cells
+        HIPCK(plan->d_Qk.ensure(F * std::max(g.nl, 1) * 8 * 8));
+        HIPCK(plan->d_Pk.ensure(F * std::max(g.nl, 1) * 8 * 8));
+        HIPCK(plan->d_Qk_img.ensure(F * std::max(g.nimg, 1) * 8 * 16));
+        HIPCK(plan->d_Pk_img.ensure(F * std::max(g.nimg, 1) * 8 * 16));
+        HIPCK(plan->d_rel.ensure((size_t)n * 16));
+    } else {
+        HIPCK(plan->d_Qk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
+        HIPCK(plan->d_Pk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
+    }
+    HIPCK(plan->d_partial.ensure(256 * 5 * 8));
+    {   // Hadamard source staging: distinct sources per chunk of targets, kept when they fit in LDS
+        const bool f32 = plan->mixed();
+        const int nth = f32 ? g.nl : g.ntar;   // 8x8 Hadamard targets
+        const int CT = hadamard_chunk_targets(f32);
+        const int nchunk = std::max(1, (nth + CT - 1) / CT);
+        plan->h_ucap = 26 * CT;
+        HIPCK(plan->h_usrc.ensure((size_t)nchunk * plan->h_ucap * 4));
+        HIPCK(plan->h_ucnt.ensure((size_t)nchunk * 4));
+        HIPCK(plan->h_tslot.ensure((size_t)26 * std::max(nth, 1) * 2));
+        plan->h_umax = 0;
+        if (nth > 0) {
+            HIPCK(launch_m2l_union(f32, nth, g.ntar, g.slot_src_T.as<int32_t>(), plan->h_ucap,
+                                   plan->h_usrc.as<int32_t>(), plan->h_ucnt.as<int32_t>(),
+                                   plan->h_tslot.as<int16_t>(), s));
+            std::vector<int32_t> cnt(nchunk);
+            HIPCK(hipMemcpyAsync(cnt.data(), plan->h_ucnt.p, (size_t)nchunk * 4, hipMemcpyDeviceToHost, s));
+            HIPCK(hipStreamSynchronize(s));
+            for (int c : cnt) plan->h_umax = std::max(plan->h_umax, c);
+        }
+        plan->h_lds = nth > 0 && hadamard_lds_bytes(f32, plan->h_umax) <= 160 * 1024;
+        if (const char* e = getenv("FMM3DPP_HADAMARD_STAGING")) plan->h_lds = plan->h_lds && atoi(e) != 0;
+    }
     HIPCK(plan->d_m2m_part.ensure((size_t)8 * std::max(1, g.max_level_cols) * Sn * 8));
     HIPCK(hipStreamSynchronize(s));
     plan->host_ms["tree_and_lists"] = g.ms_build;
@@ -307,17 +356,47 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     }
     HIPCK(hipEventRecord(plan->ev[2], s));
     // M2L (fmm.cpp:152-193, kernel.cpp:1140-1397)
-    const int ntar = g.ntar;
-    HIPCK(launch_m2l_up(o.P, Sn, ntar, g.child_slots.as<int32_t>(), plan->o_lattice.as<int32_t>(),
-                        plan->o_tw.as<double>(), qeq, plan->d_Qk.as<double>(), s));
-    HIPCK(hipEventRecord(plan->ev[3], s));
-    HIPCK(launch_hadamard8(o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8.as<double>(),
-                           plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
-    HIPCK(hipEventRecord(plan->ev[4], s));
-    HIPCK(launch_hadamard27(o.F, g.nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, ntar,
-                            plan->o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
-    HIPCK(launch_m2l_down(o.P, Sn, ntar, g.child_slots.as<int32_t>(), g.tar_scale.as<double>(),
-                          plan->o_lattice.as<int32_t>(), plan->o_tw.as<double>(), plan->d_Pk.as<double>(), pck, s));
+    // Targets = sources = the tree's non-leaf cells [0, nl), then the image cells [nl, ntar).
+    const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;
+    const int32_t* slots = g.child_slots.as<int32_t>();
+    const int32_t* inv = plan->o_invlat.as<int32_t>();
+    const double* tw = plan->o_tw.as<double>();
+    if (!plan->mixed()) {
+        HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
+        HIPCK(hipEventRecord(plan->ev[3], s));
+        if (plan->h_lds)
+            HIPCK(launch_hadamard8_lds(false, o.F, ntar, plan->h_ucap, plan->h_umax, plan->h_usrc.as<int32_t>(),
+                                       plan->h_ucnt.as<int32_t>(), plan->h_tslot.as<int16_t>(), plan->o_gk8.p,
+                                       plan->d_Qk.p, plan->d_Pk.p, s));
+        else
+            HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8.p,
+                                   plan->d_Qk.p, plan->d_Pk.p, s));
+        HIPCK(hipEventRecord(plan->ev[4], s));
+        HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
+                                plan->o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
+        HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
+    } else {
+        // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
+        HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
+        HIPCK(hipEventRecord(plan->ev[3], s));
+        if (plan->h_lds)
+            HIPCK(launch_hadamard8_lds(true, o.F, nl, plan->h_ucap, plan->h_umax, plan->h_usrc.as<int32_t>(),
+                                       plan->h_ucnt.as<int32_t>(), plan->h_tslot.as<int16_t>(), plan->o_gk8f.p,
+                                       plan->d_Qk.p, plan->d_Pk.p, s));
+        else
+            HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8f.p, plan->d_Qk.p,
+                                   plan->d_Pk.p, s));
+        HIPCK(hipEventRecord(plan->ev[4], s));
+        HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
+        if (nimg > 0) {
+            HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
+            HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, nl,
+                                    plan->o_gk27.as<double>(), plan->d_Qk_img.as<double>(),
+                                    plan->d_Pk_img.as<double>(), s));
+            HIPCK(launch_m2l_down(o.P, false, Sn, nimg, slots + (size_t)nl * 8, g.tar_scale.as<double>() + nl, inv,
+                                  tw, plan->d_Pk_img.p, pck, s));
+        }
+    }
     HIPCK(hipEventRecord(plan->ev[5], s));
     // L2L (fmm.cpp:195-229): image levels from the outermost image, then one launch per tree level
     const size_t nimgm2m = g.img_m2m.size();
@@ -346,14 +425,22 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
                           plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
     HIPCK(hipEventRecord(plan->ev[7], s));
     // P2P (fmm.cpp:259-294)
-    HIPCK(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(), g.p2p_ent.as<int32_t>(),
-                     plan->cfg.cycle, plan->d_p.as<double>(), plan->d_gx.as<double>(), plan->d_gy.as<double>(),
-                     plan->d_gz.as<double>(), s));
+    if (!plan->mixed()) {
+        HIPCK(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(),
+                         g.p2p_ent.as<int32_t>(), plan->cfg.cycle, plan->d_p.as<double>(), plan->d_gx.as<double>(),
+                         plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
+    } else {
+        HIPCK(launch_pack_rel(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, plan->d_rel.as<float4>(), s));
+        HIPCK(launch_p2p_f32(g.leaf_cell.as<int32_t>(), nleaf, cells, plan->d_rel.as<float4>(),
+                             g.p2p_off.as<int64_t>(), g.p2p_ent.as<int32_t>(), plan->cfg.cycle,
+                             plan->d_p.as<double>(), plan->d_gx.as<double>(), plan->d_gy.as<double>(),
+                             plan->d_gz.as<double>(), s));
+    }
     HIPCK(hipEventRecord(plan->ev[8], s));
     // dipole correction, 1/(4 pi), input order (fmm.cpp:75-80)
     const int nblk = 256;
     HIPCK(launch_dipole_partial(bod, n, plan->d_partial.as<double>(), nblk, s));
-    HIPCK(launch_finalize(n, g.perm.as<int32_t>(), plan->d_q.as<double>(), plan->d_p.as<double>(),
+    HIPCK(launch_finalize(n, g.perm.as<int32_t>(), bod, plan->d_p.as<double>(),
                           plan->d_gx.as<double>(), plan->d_gy.as<double>(), plan->d_gz.as<double>(),
                           plan->d_partial.as<double>(), nblk, plan->cfg.dipole_mode, plan->cfg.cycle,
                           plan->cfg.divide_4pi ? 1.0 / (4 * M_PI) : 1.0, d_phi, d_grad, s));
@@ -523,6 +610,99 @@ double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
     for (int i = 0; i < kStages; i++)
         if (s == kStageNames[i]) return el(i, i + 1);
     return -1.0;
+}
+
+int fmm3dpp_evaluate_adjoint_device(fmm3dpp_plan* plan, const double* d_q, const double* d_w, const double* d_grad,
+                                    double* d_dq, double* d_dx) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
+    if (!plan->particles) FAIL(FMM3DPP_ERR_STATE, "set_particles must precede evaluate_adjoint");
+    if (!d_q || !d_w || !d_grad || !d_dq || !d_dx) FAIL(FMM3DPP_ERR_ARG, "null array");
+    if (plan->cfg.dipole_mode == FMM3DPP_DIPOLE_REFERENCE)
+        FAIL(FMM3DPP_ERR_ARG, "the reference dipole correction (phi -= c |M|^2 / (n q_i)) has no adjoint; "
+                              "use FMM3DPP_DIPOLE_OFF or FMM3DPP_DIPOLE_PHYSICAL");
+    HIPCK(hipSetDevice(plan->device));
+    const int64_t n = plan->gs.n;
+    HIPCK(plan->d_adj_phi.ensure(n * 8));
+    HIPCK(plan->d_adj_grad.ensure(n * 24));
+    // the lattice kernel is symmetric: A^T w is the evaluation with charges w, up to the gauge term
+    int rc = fmm3dpp_evaluate_device(plan, d_w, plan->d_adj_phi.as<double>(), plan->d_adj_grad.as<double>());
+    if (rc) return rc;
+    const double cyc = plan->cfg.cycle;
+    const double c = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL ? 4 * M_PI / (3 * cyc * cyc * cyc) : 0.0;
+    const double out_scale = plan->cfg.divide_4pi ? 1.0 / (4 * M_PI) : 1.0;
+    HIPCK(launch_adjoint_combine((int)n, plan->gs.perm.as<int32_t>(), plan->bodies_dev(), plan->d_partial.as<double>(),
+                                 256, c, out_scale, d_q, d_w, d_grad, plan->d_adj_phi.as<double>(),
+                                 plan->d_adj_grad.as<double>(), d_dq, d_dx, plan->stream));
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_evaluate_adjoint(fmm3dpp_plan* plan, const double* q, const double* w, const double* grad, double* dq,
+                             double* dx) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
+    if (!plan->particles) FAIL(FMM3DPP_ERR_STATE, "set_particles must precede evaluate_adjoint");
+    if (!q || !w || !grad || !dq || !dx) FAIL(FMM3DPP_ERR_ARG, "null array");
+    HIPCK(hipSetDevice(plan->device));
+    const int64_t n = plan->gs.n;
+    hipStream_t s = plan->stream;
+    HIPCK(plan->d_qin.ensure(n * 8));
+    HIPCK(plan->d_adj_w.ensure(n * 8));
+    HIPCK(plan->d_adj_g.ensure(n * 24));
+    HIPCK(plan->d_adj_dq.ensure(n * 8));
+    HIPCK(plan->d_adj_dx.ensure(n * 24));
+    HIPCK(hipMemcpyAsync(plan->d_qin.p, q, n * 8, hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(plan->d_adj_w.p, w, n * 8, hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(plan->d_adj_g.p, grad, n * 24, hipMemcpyHostToDevice, s));
+    int rc = fmm3dpp_evaluate_adjoint_device(plan, plan->d_qin.as<double>(), plan->d_adj_w.as<double>(),
+                                             plan->d_adj_g.as<double>(), plan->d_adj_dq.as<double>(),
+                                             plan->d_adj_dx.as<double>());
+    if (rc) return rc;
+    HIPCK(hipMemcpyAsync(dq, plan->d_adj_dq.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(dx, plan->d_adj_dx.p, n * 24, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_ewald_device(void* hip_stream, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                         const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int ksize,
+                         double alpha, double cutoff, double* d_phi, double* d_grad) {
+    if (n < 1 || ntar < 1 || !d_x || !d_y || !d_z || !d_q || !d_targets || !d_phi || !d_grad) return FMM3DPP_ERR_ARG;
+    if (ksize < 1 || ksize > 23 || !(cycle > 0) || !(alpha > 0) || !(cutoff > 0) || cutoff > cycle / 2)
+        return FMM3DPP_ERR_ARG;
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    EwaldScratch ws;
+    if (ewald_device(n, d_x, d_y, d_z, d_q, ntar, d_targets, cycle, ksize, alpha, cutoff, d_phi, d_grad, ws, st) !=
+        hipSuccess)
+        return FMM3DPP_ERR_HIP;
+    return hipStreamSynchronize(st) == hipSuccess ? FMM3DPP_OK : FMM3DPP_ERR_HIP;
+}
+
+int fmm3dpp_ewald(int device, int64_t n, const double* x, const double* y, const double* z, const double* q,
+                  int32_t ntar, const int32_t* targets, double cycle, int ksize, double alpha, double cutoff,
+                  double* phi, double* grad) {
+    if (n < 1 || ntar < 1 || !x || !y || !z || !q || !targets || !phi || !grad) return FMM3DPP_ERR_ARG;
+    for (int32_t t = 0; t < ntar; t++)
+        if (targets[t] < 0 || targets[t] >= n) return FMM3DPP_ERR_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return FMM3DPP_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return FMM3DPP_ERR_DEVICE;
+    DBuf dx, dy, dz, dq, dt, dp, dg;
+    const size_t nb = (size_t)n * 8;
+    if (dx.ensure(nb) || dy.ensure(nb) || dz.ensure(nb) || dq.ensure(nb) || dt.ensure((size_t)ntar * 4) ||
+        dp.ensure((size_t)ntar * 8) || dg.ensure((size_t)ntar * 24))
+        return FMM3DPP_ERR_ALLOC;
+    if (hipMemcpy(dx.p, x, nb, hipMemcpyHostToDevice) || hipMemcpy(dy.p, y, nb, hipMemcpyHostToDevice) ||
+        hipMemcpy(dz.p, z, nb, hipMemcpyHostToDevice) || hipMemcpy(dq.p, q, nb, hipMemcpyHostToDevice) ||
+        hipMemcpy(dt.p, targets, (size_t)ntar * 4, hipMemcpyHostToDevice))
+        return FMM3DPP_ERR_HIP;
+    int rc = fmm3dpp_ewald_device(nullptr, n, dx.as<double>(), dy.as<double>(), dz.as<double>(), dq.as<double>(), ntar,
+                                  dt.as<int32_t>(), cycle, ksize, alpha, cutoff, dp.as<double>(), dg.as<double>());
+    if (rc) return rc;
+    if (hipMemcpy(phi, dp.p, (size_t)ntar * 8, hipMemcpyDeviceToHost) ||
+        hipMemcpy(grad, dg.p, (size_t)ntar * 24, hipMemcpyDeviceToHost))
+        return FMM3DPP_ERR_HIP;
+    return FMM3DPP_OK;
 }
 
 }  // extern "C"

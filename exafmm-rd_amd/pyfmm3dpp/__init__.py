@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfmm3dpp.so")
 OK = 0
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_TREE, ERR_ALLOC, ERR_HIP = -1, -2, -3, -4, -5, -6
 PREC_FP64, PREC_MIXED = 0, 1
-DIPOLE_OFF, DIPOLE_REFERENCE = 0, 1
+DIPOLE_OFF, DIPOLE_REFERENCE, DIPOLE_PHYSICAL = 0, 1, 2
 DEVICE_NONE = -1
 
 # Every entry point include/fmm3dpp.h declares.
@@ -25,6 +25,7 @@ EXPORTS = (
     "fmm3dpp_set_particles", "fmm3dpp_set_particles_device", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
+    "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
 )
 
 
@@ -55,6 +56,10 @@ def lib():
     """Load libfmm3dpp.so; raises if it was not built."""
     global _lib
     if _lib is None:
+        try:   # torch bundles its own HIP runtime (same soname): load it first so that the process has one
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing: build it with `make -C exafmm-rd_amd/csrc`")
         L = ctypes.CDLL(LIB_PATH)
@@ -80,6 +85,12 @@ def lib():
         L.fmm3dpp_get_stage_vectors.argtypes = [vp, _dp, _dp]
         L.fmm3dpp_timing_ms.argtypes = [vp, ctypes.c_char_p]
         L.fmm3dpp_timing_ms.restype = ctypes.c_double
+        L.fmm3dpp_evaluate_adjoint.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        L.fmm3dpp_evaluate_adjoint_device.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.fmm3dpp_ewald.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_int32, _ip,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.fmm3dpp_ewald_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int32, vp, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp]
         _lib = L
     return _lib
 
@@ -97,16 +108,19 @@ class LaplaceFMM:
 
     Arguments follow the reference's Argument fields (3dpp/include/argument.h:14-44): P, ncrit,
     images, cycle, dipole_correction, divide_4pi; the box is centred at `center` with half size cycle/2.
+    `dipole_mode` (DIPOLE_OFF / DIPOLE_REFERENCE / DIPOLE_PHYSICAL) overrides dipole_correction.
     """
 
     def __init__(self, P, ncrit, images, cycle, dipole_correction=1, divide_4pi=0, precision=PREC_FP64,
-                 device=0, center=(0.0, 0.0, 0.0)):
+                 device=0, center=(0.0, 0.0, 0.0), dipole_mode=None):
         cfg = Config()
         cfg.P, cfg.ncrit, cfg.images, cfg.cycle = P, ncrit, images, cycle
         for k in range(3):
             cfg.center[k] = center[k]
         cfg.precision = precision
-        cfg.dipole_mode = DIPOLE_REFERENCE if dipole_correction else DIPOLE_OFF
+        if dipole_mode is None:
+            dipole_mode = DIPOLE_REFERENCE if dipole_correction else DIPOLE_OFF
+        cfg.dipole_mode = dipole_mode
         cfg.divide_4pi = int(bool(divide_4pi))
         h = ctypes.c_void_p()
         rc = lib().fmm3dpp_plan_create(ctypes.byref(cfg), device, ctypes.byref(h))
@@ -161,6 +175,21 @@ class LaplaceFMM:
         """Device pointers (ints), ordered on the plan's stream."""
         self._check(lib().fmm3dpp_evaluate_device(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(phi_ptr),
                                                    ctypes.c_void_p(grad_ptr)))
+
+    def evaluate_adjoint(self, q, w, grad):
+        """Backward of phi = evaluate(q) for w = dL/dphi and the forward gradient `grad`:
+        returns (dL/dq, dL/dx) (fmm3dpp_evaluate_adjoint)."""
+        q, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (q, w))
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        dq = np.empty(self.n)
+        dx = np.empty((self.n, 3))
+        self._check(lib().fmm3dpp_evaluate_adjoint(self.h, _d(q), _d(w), _d(grad), _d(dq), _d(dx)))
+        return dq, dx
+
+    def evaluate_adjoint_device(self, q_ptr, w_ptr, grad_ptr, dq_ptr, dx_ptr):
+        """Device pointers (ints), ordered on the plan's stream."""
+        self._check(lib().fmm3dpp_evaluate_adjoint_device(self.h, *(ctypes.c_void_p(p) for p in
+                                                                    (q_ptr, w_ptr, grad_ptr, dq_ptr, dx_ptr))))
 
     def set_stream(self, stream_ptr):
         self._check(lib().fmm3dpp_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
@@ -219,6 +248,23 @@ class LaplaceFMM:
         pck = np.empty((c["n_cells"], c["S"]))
         self._check(lib().fmm3dpp_get_stage_vectors(self.h, _d(qeq), _d(pck)))
         return qeq, pck
+
+
+def ewald(x, y, z, q, cycle, ksize, alpha=None, cutoff=None, targets=None, device=0):
+    """Tin-foil Ewald sum on the GPU (fmm3dpp_ewald); the reference's defaults alpha = ksize / cycle,
+    cutoff = cycle / 2 (3dpp/src/ewald.cpp:9-14).  Returns phi[ntar], grad[ntar, 3]."""
+    x, y, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+    n = len(x)
+    alpha = ksize / cycle if alpha is None else alpha
+    cutoff = cycle / 2 if cutoff is None else cutoff
+    targets = np.arange(n, dtype=np.int32) if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+    phi = np.empty(len(targets))
+    grad = np.empty((len(targets), 3))
+    rc = lib().fmm3dpp_ewald(device, n, _d(x), _d(y), _d(z), _d(q), len(targets), _i(targets), cycle, ksize,
+                             alpha, cutoff, _d(phi), _d(grad))
+    if rc != OK:
+        raise FMMError(rc, "fmm3dpp_ewald failed")
+    return phi, grad
 
 
 OPERATOR_NAMES = (["UT", "VSinv", "UT_l2p", "VSinv_l2p", "GK8"] + [f"M2M{o}" for o in range(8)]

@@ -41,6 +41,13 @@ extern "C" {
 
 #define FMM3DPP_DIPOLE_OFF 0      /* no periodic dipole correction                                */
 #define FMM3DPP_DIPOLE_REFERENCE 1 /* the reference's correction (src/kernel.cpp:1580-1596)      */
+#define FMM3DPP_DIPOLE_PHYSICAL 2 /* infinite-lattice (tin-foil Ewald) gauge, no counterpart in the
+                                   * reference: with M = sum q_j x_j, V = cycle^3,
+                                   *   phi_i  -= (4 pi / 3V) M.x_i - (2 pi / 3V) sum_j q_j |x_j|^2
+                                   *   grad_i -= (4 pi / 3V) M
+                                   * which turns the lattice sum over the (3^images)^3 cube into the
+                                   * Ewald potential of a neutral system (the cube's dipole sum is
+                                   * conditionally convergent; de Leeuw, Perram and Smith 1980).    */
 
 #define FMM3DPP_DEVICE_NONE (-1)  /* structure-only plan: tree, lists and operators on the host; */
                                   /* evaluate returns FMM3DPP_ERR_DEVICE                          */
@@ -116,6 +123,35 @@ int fmm3dpp_set_operator(fmm3dpp_plan* plan, const char* name, const double* in,
 
 /* Per-cell equivalent densities and check potentials [n_cells][S] after the last evaluate. */
 int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_check);
+
+/* ---- adjoint (no counterpart in the reference: the backward of phi = evaluate(q)) ----
+ * For a loss L(phi) with w = dL/dphi, and grad = the forward gradient grad(phi) of the same plan and
+ * particles, returns dq = dL/dq and dx = dL/dx (n x 3, input order).  The Laplace lattice kernel is
+ * symmetric, so A^T w is one more evaluation with charges w (plus the gauge term in
+ * FMM3DPP_DIPOLE_PHYSICAL); dx combines w grad(phi) with q grad(A w).  The energy E = 1/2 q.phi has
+ * dE/dq = phi and dE/dx = q grad(phi).  Losses of grad(phi) (forces) are not covered.  Refused
+ * (FMM3DPP_ERR_ARG) for FMM3DPP_DIPOLE_REFERENCE, whose potential term -c |M|^2 / (n q_i) has none.
+ * The _device form takes device arrays, ordered on the plan's stream. */
+int fmm3dpp_evaluate_adjoint(fmm3dpp_plan* plan, const double* q, const double* w, const double* grad, double* dq,
+                             double* dx);
+int fmm3dpp_evaluate_adjoint_device(fmm3dpp_plan* plan, const double* d_q, const double* d_w, const double* d_grad,
+                                    double* d_dq, double* d_dx);
+
+/* ---- Ewald summation (the reference's EwaldSolver(bodies, args).solve(), include/ewald.h:20-21,
+ *      src/ewald.cpp:6-203), float64 on the GPU, at ntar target bodies ----
+ * Tin-foil boundary:  phi_t = sum_j q_j erfc(alpha r)/r (minimum image, 0 < r < cutoff)
+ *   + sum_{0 < max|k_i| <= ksize} (4 pi / L^3) exp(-k^2 / 4 alpha^2) / k^2 Re(S(k) e^{i k.x_t})
+ *   - 2 alpha / sqrt(pi) q_t,  k = 2 pi (a, b, c) / cycle.  grad as phi.  ksize <= 23.
+ * The reference fixes alpha = ksize / cycle and cutoff = cycle / 2 (ewald.cpp:9-14).
+ * fmm3dpp_ewald takes host arrays (targets: indices into the n bodies; phi[ntar], grad[ntar][3])
+ * and runs on `device`; fmm3dpp_ewald_device takes device arrays on the current device and is
+ * ordered on hip_stream (it returns after the stream has finished). */
+int fmm3dpp_ewald(int device, int64_t n, const double* x, const double* y, const double* z, const double* q,
+                  int32_t ntar, const int32_t* targets, double cycle, int ksize, double alpha, double cutoff,
+                  double* phi, double* grad);
+int fmm3dpp_ewald_device(void* hip_stream, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                         const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int ksize,
+                         double alpha, double cutoff, double* d_phi, double* d_grad);
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),

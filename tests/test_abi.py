@@ -48,6 +48,17 @@ def test_invalid_config_rejected(bad):
     assert rc == pyfmm3dpp.ERR_ARG and not h.value
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_both_precisions_accepted(prec):
+    """FMM3DPP_PREC_FP64 and FMM3DPP_PREC_MIXED both validate; the structure is the same for both."""
+    import pyfmm3dpp
+    h = ctypes.c_void_p()
+    rc = pyfmm3dpp.lib().fmm3dpp_plan_create(ctypes.byref(_cfg(precision=prec)), pyfmm3dpp.DEVICE_NONE,
+                                             ctypes.byref(h))
+    assert rc == pyfmm3dpp.OK and h.value
+    assert pyfmm3dpp.lib().fmm3dpp_plan_destroy(h) == pyfmm3dpp.OK
+
+
 def test_null_arguments():
     import pyfmm3dpp
     L = pyfmm3dpp.lib()

@@ -1,0 +1,88 @@
+"""Mixed-precision path (FMM3DPP_PREC_MIXED) against the float64 path on the same plan inputs.
+
+The mixed path runs the M2L spectra and 8x8 Hadamard (matrix cores) and the P2P near field in
+float32 and keeps float64 for the check potentials, every pseudo-inverse apply, M2M/L2L, the L2P
+evaluation and the image lattice.  Both paths share the tree, lists and operators, so the
+difference is the float32 rounding alone.  Bounds: 2e-6 relative L2 for the potential, 1e-5 for
+the gradient.  The gradient's share comes from the P2P: float32 leaf-relative coordinates carry
+an absolute error of ~6e-8 of the leaf size, which is a large relative error in the displacement
+of the closest pairs, and 1/r^2 weighs those most.  A host emulation of the float32 P2P alone
+gives 4.2e-6 (N=65536, P=6, ncrit=64) and 8.3e-6 (N=60000, ncrit=128), as measured here.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PHI_TOL, GRAD_TOL = 2e-6, 1e-5
+
+
+@pytest.fixture(scope="module")
+def fmm():
+    import pyfmm3dpp
+    pyfmm3dpp.lib()
+    return pyfmm3dpp
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _uniform(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    return x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), q
+
+
+def _clustered(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.concatenate([rng.normal(0.1, 0.05, (n // 2, 3)), rng.random((n - n // 2, 3)) - 0.5])
+    x = (x + 0.5) % 1.0 - 0.5
+    q = rng.choice([-1.0, 1.0], n)
+    q -= q.mean()
+    return x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), q
+
+
+def _both(fmm, x, y, z, q, P, ncrit, images, dipole=1):
+    print(f"case n={len(x)} P={P} ncrit={ncrit} images={images}", flush=True)
+    out = []
+    for prec in (fmm.PREC_FP64, fmm.PREC_MIXED):
+        plan = fmm.LaplaceFMM(P, ncrit, images, 1.0, dipole_correction=dipole, precision=prec, device=0)
+        plan.precompute()
+        plan.set_particles(x, y, z)
+        out.append(plan.evaluate(q))
+    (p64, g64), (pmx, gmx) = out
+    ep, eg = _rel_l2(pmx, p64), _rel_l2(gmx, g64)
+    print(f"  mixed vs float64: rel L2 phi {ep:.2e} grad {eg:.2e}", flush=True)
+    assert np.all(np.isfinite(pmx)) and np.all(np.isfinite(gmx))
+    return ep, eg
+
+
+@pytest.mark.parametrize("P,images", [(4, 0), (4, 1), (6, 2), (8, 3)])
+def test_mixed_vs_fp64_small(fmm, P, images):
+    x, y, z, q = _uniform(8192, 2000 + P + images)
+    ep, eg = _both(fmm, x, y, z, q, P, 64, images)
+    assert ep <= PHI_TOL and eg <= GRAD_TOL
+
+
+def test_mixed_vs_fp64_config1(fmm):
+    """N=65536, P=6, ncrit=64, images=2 (the CPU-runnable configuration)."""
+    x, y, z, q = _uniform(65536, 42)
+    ep, eg = _both(fmm, x, y, z, q, 6, 64, 2)
+    assert ep <= PHI_TOL and eg <= GRAD_TOL
+
+
+def test_mixed_vs_fp64_adaptive(fmm):
+    """Clustered bodies: deep adaptive tree, uneven leaves (1..ncrit bodies), partial 8x8 patterns."""
+    x, y, z, q = _clustered(30000, 7)
+    ep, eg = _both(fmm, x, y, z, q, 6, 32, 2)
+    assert ep <= PHI_TOL and eg <= GRAD_TOL
+
+
+def test_mixed_large_leaves(fmm):
+    """ncrit=128: leaves above one wave's 64 target slots take the chunked P2P loop."""
+    x, y, z, q = _uniform(60000, 11)
+    ep, eg = _both(fmm, x, y, z, q, 4, 128, 1)
+    assert ep <= PHI_TOL and eg <= GRAD_TOL
