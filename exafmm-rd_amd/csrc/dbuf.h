@@ -49,11 +49,16 @@ struct DBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Copies a host vector into d, ordered on st, and returns after the copy has completed: callers
+// pass temporaries that are destroyed right after the call, and a pageable-memory copy may read
+// its source only when the stream reaches it.
 template <class T>
 hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
     hipError_t e = d.ensure(h.size() * sizeof(T));
     if (e != hipSuccess || h.empty()) return e;
-    return hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
 }
 
 }  // namespace fmm3dpp

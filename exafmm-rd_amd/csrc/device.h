@@ -55,20 +55,9 @@ hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, cons
 // inv_lat maps the P^3 corner of the convolution grid ((k * P + j) * P + i) to a surface point or -1.
 hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
                          const double* twiddle, const double* qeq, void* Qk, hipStream_t st);
-// targets t < ntar, slot_src_T rows of stride ntab
+// targets t < ntar, slot_src_T rows of stride ntab (-1: no source)
 hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
                             const void* Qk, void* Pk, hipStream_t st);
-// Hadamard with per-chunk source staging in LDS.  Chunks are hadamard_chunk_targets(f32) consecutive
-// targets; u_src[chunk * ucap + u] (ucap >= 26 * chunk targets) lists a chunk's distinct sources,
-// u_cnt[chunk] their number, t_slot[p * ntar + t] the position of (t, p)'s source or -1.  umax =
-// max u_cnt; the Hadamard needs hadamard_lds_bytes(f32, umax) <= 160 KiB of LDS.
-int hadamard_chunk_targets(bool f32);
-size_t hadamard_lds_bytes(bool f32, int umax);
-hipError_t launch_m2l_union(bool f32, int ntar, int ntab, const int32_t* slot_src_T, int ucap, int32_t* u_src,
-                            int32_t* u_cnt, int16_t* t_slot, hipStream_t st);
-hipError_t launch_hadamard8_lds(bool f32, int F, int ntar, int ucap, int umax, const int32_t* u_src,
-                                const int32_t* u_cnt, const int16_t* t_slot, const void* gk8, const void* Qk,
-                                void* Pk, hipStream_t st);
 // float64 1x27 image lattice; img_src_T rows have stride ntab, Qk/Pk hold the parents from index base on
 hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,
                              const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);

@@ -130,15 +130,17 @@ __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
 }
 
 // Hadamard 8x8 (src/kernel.cpp:1012-1082, include/c2c.h:201-296) on the matrix cores.  Targets
-// t < ntar, source table rows slot_src_T[p * ntab + t].  For one frequency f:  Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
+// t < ntar, source table rows slot_src_T[p * ntab + t] (-1: no source).  For one frequency f:
+//   Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
 // Each complex 8x8 G_p acts as the real 16x16 matrix M on interleaved (re, im) vectors:
 //   M[2it][2is] = M[2it+1][2is+1] = Re G[is][it],  M[2it][2is+1] = -Im G,  M[2it+1][2is] = Im G.
 // A wave multiplies M (A operand, rows = output reals) by 16 targets' source vectors (B operand,
 // columns = targets) with four K=4 steps; lane (n, kr) feeds reals 4kr..4kr+3 of target n's source
 // vector, so the K order is permuted (step kk, lane group kr -> real 4kr + kk) identically in A and
 // B.  The A rows are permuted too, so that output register i of lane (n, kr) is real 4kr + i of
-// target n whatever the accumulator layout of the type: A row d_row(g, i) holds output real 4g + i.  TT tiles of 16 targets per wave, 4 waves per workgroup, one frequency per workgroup; blocks
-// are remapped so that consecutive frequency slices run on one XCD.
+// target n whatever the accumulator layout of the type: A row d_row(g, i) holds output real 4g + i.
+// TT tiles of 16 targets per wave, 4 waves per workgroup, one frequency per workgroup; blocks are
+// remapped so that consecutive frequency slices run on one XCD.
 template <typename T, int TT>
 __global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int ntab, int nchunk, int total,
                                                         const int32_t* __restrict__ slot_src_T,
@@ -163,184 +165,6 @@ __global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int nta
             if (d_row<T>(g, i) == n) o = 4 * g + i;
     const int itA = o >> 1, imA = o & 1;
     const int tbase = (chunk * 4 + wave) * 16 * TT;
-    // Software pipeline over the 26 patterns: source indices two patterns ahead, source vectors one
-    // pattern ahead, all loads unconditional (clamped addresses, zeroed afterwards), so that the
-    // matrix cores never wait on a dependent chain of memory round trips.
-    int tcl[TT];
-    bool tin[TT];
-#pragma unroll
-    for (int j = 0; j < TT; j++) {
-        const int t = tbase + 16 * j + n;
-        tin[j] = t < ntar;
-        tcl[j] = tin[j] ? t : ntar - 1;
-    }
-    auto load_idx = [&](int p, int* sidx) {
-#pragma unroll
-        for (int j = 0; j < TT; j++) {
-            const int v = slot_src_T[(size_t)p * ntab + tcl[j]];
-            sidx[j] = tin[j] ? v : -1;
-        }
-    };
-    auto load_vec = [&](const int* sidx, vec4<T>* raw) {   // zeroed for sidx < 0 only at use
-#pragma unroll
-        for (int j = 0; j < TT; j++)
-            raw[j] = *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)(sidx[j] >= 0 ? sidx[j] : 0) * F + f) * 16 + 4 * kr);
-    };
-    vec4<T> acc[TT], braw[TT], nraw[TT];
-    int s0[TT], s1[TT], s2[TT];
-#pragma unroll
-    for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
-    load_idx(0, s0);
-    load_vec(s0, braw);
-    load_idx(1, s1);
-#pragma unroll   // the buffer rotation below becomes register renaming
-    for (int p = 0; p < 26; p++) {
-        // keep the stages in place: the scheduler would otherwise hoist the next pattern's address
-        // arithmetic, and with it a wait on the loads just issued, into this one
-        __builtin_amdgcn_sched_barrier(0);
-        if (p + 1 < 26) load_vec(s1, nraw);
-        if (p + 2 < 26) load_idx(p + 2, s2);
-        const T* gp = gs + p * 128;
-        const T g0r = gp[((2 * kr) * 8 + itA) * 2], g0i = gp[((2 * kr) * 8 + itA) * 2 + 1];
-        const T g1r = gp[((2 * kr + 1) * 8 + itA) * 2], g1i = gp[((2 * kr + 1) * 8 + itA) * 2 + 1];
-        const T a0 = imA ? g0i : g0r, a1 = imA ? g0r : -g0i, a2 = imA ? g1i : g1r, a3 = imA ? g1r : -g1i;
-        vec4<T> bv[TT];
-#pragma unroll
-        for (int j = 0; j < TT; j++) bv[j] = s0[j] >= 0 ? braw[j] : vec4<T>{0, 0, 0, 0};
-#pragma unroll
-        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a0, bv[j][0], acc[j]);
-#pragma unroll
-        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a1, bv[j][1], acc[j]);
-#pragma unroll
-        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a2, bv[j][2], acc[j]);
-#pragma unroll
-        for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a3, bv[j][3], acc[j]);
-#pragma unroll
-        for (int j = 0; j < TT; j++) {
-            braw[j] = nraw[j];
-            s0[j] = s1[j];
-            s1[j] = s2[j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < TT; j++) {
-        const int t = tbase + 16 * j + n;
-        if (t < ntar) *reinterpret_cast<vec4<T>*>(Pk + ((size_t)t * F + f) * 16 + 4 * kr) = acc[j];
-    }
-}
-
-// Source staging for the Hadamard: per chunk of CT consecutive targets (Morton-contiguous, so a
-// compact block of cells), the sorted distinct sources of its 26 x CT (target, pattern) pairs and,
-// per pair, the source's position in that list (-1: none).  One workgroup per chunk: keys sorted
-// bitonically in LDS, compacted with a block scan, positions found by binary search.
-constexpr int kUnionKeys = 8192;   // >= 26 * CT rounded up to a power of two, CT <= 256
-__global__ __launch_bounds__(256) void k_m2l_union(int ntar, int ntab, int CT, int nkeys,
-                                                   const int32_t* __restrict__ slot_src_T, int ucap,
-                                                   int32_t* __restrict__ u_src, int32_t* __restrict__ u_cnt,
-                                                   int16_t* __restrict__ t_slot) {
-    __shared__ int32_t key[kUnionKeys];
-    __shared__ int32_t uniq[26 * 256];
-    __shared__ int32_t scan[256];
-    const int chunk = blockIdx.x, tid = threadIdx.x;
-    for (int e = tid; e < nkeys; e += 256) key[e] = INT32_MAX;
-    __syncthreads();
-    for (int e = tid; e < 26 * CT; e += 256) {
-        const int p = e / CT, t = chunk * CT + e % CT;
-        const int sidx = t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
-        key[e] = sidx >= 0 ? sidx : INT32_MAX;
-    }
-    __syncthreads();
-    for (int k = 2; k <= nkeys; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < nkeys; i += 256) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const int a = key[i], b = key[l];
-                    if ((a > b) == ((i & k) == 0)) {
-                        key[i] = b;
-                        key[l] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    const int per = nkeys / 256, base = tid * per;
-    int cnt = 0;
-    for (int i = base; i < base + per; i++)
-        cnt += key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1]);
-    scan[tid] = cnt;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {   // inclusive scan
-        const int v = tid >= d ? scan[tid - d] : 0;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
-    }
-    int o = scan[tid] - cnt;
-    for (int i = base; i < base + per; i++)
-        if (key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1])) {
-            uniq[o] = key[i];
-            u_src[(size_t)chunk * ucap + o] = key[i];
-            o++;
-        }
-    __syncthreads();
-    const int U = scan[255];
-    if (tid == 0) u_cnt[chunk] = U;
-    for (int e = tid; e < 26 * CT; e += 256) {
-        const int p = e / CT, t = chunk * CT + e % CT;
-        if (t >= ntar) continue;
-        const int sidx = slot_src_T[(size_t)p * ntab + t];
-        int slot = -1;
-        if (sidx >= 0) {
-            int lo = 0, hi = U;   // first uniq >= sidx
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (uniq[mid] < sidx) lo = mid + 1;
-                else hi = mid;
-            }
-            slot = lo;
-        }
-        t_slot[(size_t)p * ntar + t] = (int16_t)slot;
-    }
-}
-
-// Hadamard 8x8 with the chunk's source vectors of frequency f staged in LDS (each read once from
-// HBM/L2 per workgroup instead of once per (target, pattern) pair).  Same MFMA scheme as
-// k_hadamard8_mfma; CT = 64 TT targets per workgroup (4 waves x TT tiles of 16).
-template <typename T, int TT>
-__global__ __launch_bounds__(256) void k_hadamard8_lds(int F, int ntar, int nchunk, int total, int ucap,
-                                                       const int32_t* __restrict__ u_src,
-                                                       const int32_t* __restrict__ u_cnt,
-                                                       const int16_t* __restrict__ t_slot,
-                                                       const T* __restrict__ gk8, const T* __restrict__ Qk,
-                                                       T* __restrict__ Pk) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* gs = reinterpret_cast<T*>(lds_raw);   // [26][128]
-    T* qs = gs + 26 * 128;                   // [U][16]
-    const int per = (total + 7) / 8;
-    const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
-    if (Lb >= total) return;                                    // uniform over the workgroup
-    const int f = Lb / nchunk, chunk = Lb % nchunk;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < 26 * 128; e += 256) {
-        const int p = e >> 7, m = e & 127;
-        gs[e] = gk8[((size_t)p * F + f) * 128 + m];
-    }
-    const int U = u_cnt[chunk];
-    const int32_t* us = u_src + (size_t)chunk * ucap;
-    for (int e = tid; e < 4 * U; e += 256) {
-        const int u = e >> 2, part = e & 3;
-        *reinterpret_cast<vec4<T>*>(qs + u * 16 + 4 * part) =
-            *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)us[u] * F + f) * 16 + 4 * part);
-    }
-    __syncthreads();
-    const int n = lane & 15, kr = lane >> 4;
-    int o = n;
-    for (int g = 0; g < 4; g++)
-        for (int i = 0; i < 4; i++)
-            if (d_row<T>(g, i) == n) o = 4 * g + i;
-    const int itA = o >> 1, imA = o & 1;
-    const int tbase = (chunk * 4 + wave) * 16 * TT;
     vec4<T> acc[TT];
 #pragma unroll
     for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
@@ -353,9 +177,9 @@ __global__ __launch_bounds__(256) void k_hadamard8_lds(int F, int ntar, int nchu
 #pragma unroll
         for (int j = 0; j < TT; j++) {
             const int t = tbase + 16 * j + n;
-            const int sl = t < ntar ? t_slot[(size_t)p * ntar + t] : -1;
-            if (sl >= 0)
-                bv[j] = *reinterpret_cast<const vec4<T>*>(qs + sl * 16 + 4 * kr);
+            const int s = t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
+            if (s >= 0)
+                bv[j] = *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)s * F + f) * 16 + 4 * kr);
             else
                 bv[j] = vec4<T>{0, 0, 0, 0};
         }
@@ -576,27 +400,6 @@ hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, con
     return hipGetLastError();
 }
 
-template <typename T, int TT>
-hipError_t hadamard8_lds_t(int F, int ntar, int ucap, int umax, const int32_t* u_src, const int32_t* u_cnt,
-                           const int16_t* t_slot, const void* gk8, const void* Qk, void* Pk, hipStream_t st) {
-    const int CT = 64 * TT;
-    const int nchunk = (ntar + CT - 1) / CT;
-    const int total = nchunk * F;
-    const int per = (total + 7) / 8;
-    const size_t lds = (size_t)(26 * 128 + 16 * (size_t)umax) * sizeof(T);
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_hadamard8_lds<T, TT>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
-    k_hadamard8_lds<T, TT><<<per * 8, 256, lds, st>>>(F, ntar, nchunk, total, ucap, u_src, u_cnt, t_slot,
-                                                      static_cast<const T*>(gk8), static_cast<const T*>(Qk),
-                                                      static_cast<T*>(Pk));
-    return hipGetLastError();
-}
-
 #define FMM3DPP_P_CASES(M) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12)
 
 }  // namespace
@@ -644,30 +447,6 @@ hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_
 #undef DNCASE
         default: return hipErrorInvalidValue;
     }
-}
-
-int hadamard_chunk_targets(bool f32) { return f32 ? 256 : 128; }
-
-size_t hadamard_lds_bytes(bool f32, int umax) { return (size_t)(26 * 128 + 16 * (size_t)umax) * (f32 ? 4 : 8); }
-
-hipError_t launch_m2l_union(bool f32, int ntar, int ntab, const int32_t* slot_src_T, int ucap, int32_t* u_src,
-                            int32_t* u_cnt, int16_t* t_slot, hipStream_t st) {
-    if (ntar <= 0) return hipSuccess;
-    const int CT = hadamard_chunk_targets(f32);
-    int nkeys = 256;
-    while (nkeys < 26 * CT) nkeys <<= 1;
-    if (nkeys > kUnionKeys || ucap < 26 * CT) return hipErrorInvalidValue;
-    k_m2l_union<<<(ntar + CT - 1) / CT, 256, 0, st>>>(ntar, ntab, CT, nkeys, slot_src_T, ucap, u_src, u_cnt, t_slot);
-    return hipGetLastError();
-}
-
-hipError_t launch_hadamard8_lds(bool f32, int F, int ntar, int ucap, int umax, const int32_t* u_src,
-                                const int32_t* u_cnt, const int16_t* t_slot, const void* gk8, const void* Qk,
-                                void* Pk, hipStream_t st) {
-    if (ntar <= 0) return hipSuccess;
-    if (hadamard_lds_bytes(f32, umax) > 160 * 1024) return hipErrorInvalidValue;
-    return f32 ? hadamard8_lds_t<float, 4>(F, ntar, ucap, umax, u_src, u_cnt, t_slot, gk8, Qk, Pk, st)
-               : hadamard8_lds_t<double, 2>(F, ntar, ucap, umax, u_src, u_cnt, t_slot, gk8, Qk, Pk, st);
 }
 
 }  // namespace fmm3dpp

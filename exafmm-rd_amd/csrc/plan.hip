@@ -46,9 +46,6 @@ struct fmm3dpp_plan {
     DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
     DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
     DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
-    DBuf h_usrc, h_ucnt, h_tslot;     // Hadamard source staging (per chunk of targets)
-    int h_ucap = 0, h_umax = 0;
-    bool h_lds = false;
     DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_all, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk8f, o_gk27, o_usurf, o_invlat,
         o_tw;
 
@@ -169,29 +166,6 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
         HIPCK(plan->d_Pk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
     }
     HIPCK(plan->d_partial.ensure(256 * 5 * 8));
-    {   // Hadamard source staging: distinct sources per chunk of targets, kept when they fit in LDS
-        const bool f32 = plan->mixed();
-        const int nth = f32 ? g.nl : g.ntar;   // 8x8 Hadamard targets
-        const int CT = hadamard_chunk_targets(f32);
-        const int nchunk = std::max(1, (nth + CT - 1) / CT);
-        plan->h_ucap = 26 * CT;
-        HIPCK(plan->h_usrc.ensure((size_t)nchunk * plan->h_ucap * 4));
-        HIPCK(plan->h_ucnt.ensure((size_t)nchunk * 4));
-        HIPCK(plan->h_tslot.ensure((size_t)26 * std::max(nth, 1) * 2));
-        plan->h_umax = 0;
-        if (nth > 0) {
-            HIPCK(launch_m2l_union(f32, nth, g.ntar, g.slot_src_T.as<int32_t>(), plan->h_ucap,
-                                   plan->h_usrc.as<int32_t>(), plan->h_ucnt.as<int32_t>(),
-                                   plan->h_tslot.as<int16_t>(), s));
-            std::vector<int32_t> cnt(nchunk);
-            HIPCK(hipMemcpyAsync(cnt.data(), plan->h_ucnt.p, (size_t)nchunk * 4, hipMemcpyDeviceToHost, s));
-            HIPCK(hipStreamSynchronize(s));
-            for (int c : cnt) plan->h_umax = std::max(plan->h_umax, c);
-        }
-        plan->h_lds = nth > 0 && hadamard_lds_bytes(f32, plan->h_umax) <= 160 * 1024;
-        if (const char* e = getenv("FMM3DPP_HADAMARD_STAGING")) plan->h_lds = plan->h_lds && atoi(e) != 0;
-    }
-    HIPCK(plan->d_m2m_part.ensure((size_t)8 * std::max(1, g.max_level_cols) * Sn * 8));
     HIPCK(hipStreamSynchronize(s));
     plan->host_ms["tree_and_lists"] = g.ms_build;
     return FMM3DPP_OK;
@@ -364,13 +338,8 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     if (!plan->mixed()) {
         HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        if (plan->h_lds)
-            HIPCK(launch_hadamard8_lds(false, o.F, ntar, plan->h_ucap, plan->h_umax, plan->h_usrc.as<int32_t>(),
-                                       plan->h_ucnt.as<int32_t>(), plan->h_tslot.as<int16_t>(), plan->o_gk8.p,
-                                       plan->d_Qk.p, plan->d_Pk.p, s));
-        else
-            HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8.p,
-                                   plan->d_Qk.p, plan->d_Pk.p, s));
+        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8.p, plan->d_Qk.p,
+                               plan->d_Pk.p, s));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                 plan->o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
@@ -379,13 +348,8 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
         // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
         HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        if (plan->h_lds)
-            HIPCK(launch_hadamard8_lds(true, o.F, nl, plan->h_ucap, plan->h_umax, plan->h_usrc.as<int32_t>(),
-                                       plan->h_ucnt.as<int32_t>(), plan->h_tslot.as<int16_t>(), plan->o_gk8f.p,
-                                       plan->d_Qk.p, plan->d_Pk.p, s));
-        else
-            HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8f.p, plan->d_Qk.p,
-                                   plan->d_Pk.p, s));
+        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8f.p, plan->d_Qk.p,
+                               plan->d_Pk.p, s));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
         if (nimg > 0) {
