@@ -7,8 +7,13 @@
 // 3dpp/src/argument.cpp:10-69); Body3 keeps idx, q, p, x, f with f = grad(phi) (3dpp/include/body.h:8-15).
 // Errors raise std::runtime_error instead of the reference's exit(1) (3dpp/include/type.h:42-49).
 #pragma once
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,9 +41,98 @@ struct Body3 {
 };
 using Bodies3 = std::vector<Body3>;
 
+inline int verbose = 0;   // the reference's global rtfmm::verbose (src/argument.cpp:5)
+
 class Argument {
 public:
     Argument() { finalize(); }
+    // Flags and defaults of the reference's command line (src/argument.cpp:7-70), plus --precision
+    // (0 float64, 1 mixed) and --device (HIP device).  Long flags take "--name value" or
+    // "--name=value"; short flags "-P 8" or "-P8".
+    Argument(int argc, char* argv[]) {
+        std::string algo = "fde";
+        for (int i = 1; i < argc; i++) {
+            std::string a = argv[i], key, val;
+            bool has_val = false;
+            if (a.rfind("--", 0) == 0) {
+                key = a.substr(2);
+                size_t eq = key.find('=');
+                if (eq != std::string::npos) { val = key.substr(eq + 1); key = key.substr(0, eq); has_val = true; }
+            } else if (a.size() >= 2 && a[0] == '-') {
+                static const char* shorts[][2] = {{"P", "P"}, {"n", "nbody"}, {"m", "ncrit"}, {"t", "timing"},
+                                                  {"i", "images"}, {"v", "verbose"}, {"c", "cycle"},
+                                                  {"a", "algorithm"}};
+                for (auto& sp : shorts)
+                    if (a[1] == sp[0][0]) key = sp[1];
+                if (key.empty()) throw std::runtime_error("unknown option " + a);
+                if (a.size() > 2) { val = a.substr(2); has_val = true; }
+            } else {
+                throw std::runtime_error("unexpected argument " + a);
+            }
+            if (!has_val) {
+                if (i + 1 >= argc) throw std::runtime_error("option --" + key + " needs a value");
+                val = argv[++i];
+            }
+            auto I = [&]() { return std::atoi(val.c_str()); };
+            if (key == "P") P = I();
+            else if (key == "nbody") n = I();
+            else if (key == "ncrit") ncrit = I();
+            else if (key == "timing") timing = I();
+            else if (key == "images") images = I();
+            else if (key == "verbose") verbose = I();
+            else if (key == "cycle") cycle = std::atof(val.c_str());
+            else if (key == "num_compare") num_compare = I();
+            else if (key == "ewald_ksize") ewald_ksize = I();
+            else if (key == "th_num") th_num = I();
+            else if (key == "seed") seed = I();
+            else if (key == "algorithm") algo = val;
+            else if (key == "check_tree") check_tree = I();
+            else if (key == "use_fft") use_fft = I();
+            else if (key == "use_precompute") use_precompute = I();
+            else if (key == "use_simd") use_simd = I();
+            else if (key == "dipole_correction") dipole_correction = I();
+            else if (key == "zero_netcharge") zero_netcharge = I();
+            else if (key == "print_body_number") print_body_number = I();
+            else if (key == "divide_4pi") divide_4pi = I();
+            else if (key == "setting_t") setting_t = I();
+            else if (key == "precision") precision = I();
+            else if (key == "device") device = I();
+            else throw std::runtime_error("unknown option --" + key);
+        }
+        if (P < 2 || P > 32) throw std::runtime_error("option value is invalid: --P=" + std::to_string(P));
+        if (images < 0 || images > 20) throw std::runtime_error("option value is invalid: --images=" + std::to_string(images));
+        if (num_compare == -1) num_compare = n;
+        enable_fmm = algo.find('f') != std::string::npos;
+        enable_direct = algo.find('d') != std::string::npos;
+        enable_ewald = algo.find('e') != std::string::npos;
+        finalize();
+    }
+    // The reference's parameter block (src/argument.cpp:74-101).
+    void show() const {
+        printf("input parameters=[\n");
+        printf("  %-20s = %d\n", "P", P);
+        printf("  %-20s = %d\n", "n", n);
+        printf("  %-20s = %d\n", "ncrit", ncrit);
+        printf("  %-20s = %d\n", "images", images);
+        printf("  %-20s = %.4f\n", "cycle", cycle);
+        printf("  %-20s = %d\n", "ewald_ksize", ewald_ksize);
+        printf("  %-20s = (%d,%d,%d)\n", "(f,d,e)", enable_fmm, enable_direct, enable_ewald);
+        printf("  %-20s = %d\n", "num_compare", num_compare);
+        printf("  %-20s = %d\n", "th_num", th_num);
+        printf("  %-20s = %d\n", "seed", seed);
+        printf("  %-20s = %d\n", "check_tree", check_tree);
+        printf("  %-20s = %d\n", "timing", timing);
+        printf("  %-20s = %d\n", "verbose", verbose);
+        printf("  %-20s = %d\n", "use_fft", use_fft);
+        printf("  %-20s = %d\n", "use_precompute", use_precompute);
+        printf("  %-20s = %d\n", "use_simd", use_simd);
+        printf("  %-20s = %d\n", "dipole_correction", dipole_correction);
+        printf("  %-20s = %d\n", "zero_netcharge", zero_netcharge);
+        printf("  %-20s = %d\n", "print_body_number", print_body_number);
+        printf("  %-20s = %d\n", "divide_4pi", divide_4pi);
+        printf("  %-20s = %d\n", "setting_t", setting_t);
+        printf("]\n\n");
+    }
     vec3r x;
     real r = M_PI;
     int n = 1000;
@@ -57,7 +151,8 @@ public:
     int divide_4pi = 0;
     int enable_fmm = 1, enable_direct = 0, enable_ewald = 1;
     int check_tree = 0, use_fft = 1, use_precompute = 1, use_simd = 1, setting_t = 0;
-    int device = 0;   // HIP device of the plan (no counterpart in the reference)
+    int device = 0;      // HIP device of the plan (no counterpart in the reference)
+    int precision = 0;   // FMM3DPP_PREC_* (no counterpart in the reference)
     // argument.cpp:61-69: box centre 0, half size cycle/2; setting_t switches off dipole/zero-net and divides by 4 pi
     void finalize() {
         x = vec3r(0, 0, 0);
@@ -76,7 +171,7 @@ public:
         cfg.images = args_.images;
         cfg.cycle = args_.cycle;
         cfg.center[0] = args_.x[0]; cfg.center[1] = args_.x[1]; cfg.center[2] = args_.x[2];
-        cfg.precision = FMM3DPP_PREC_FP64;
+        cfg.precision = args_.precision;
         cfg.dipole_mode = args_.dipole_correction ? FMM3DPP_DIPOLE_REFERENCE : FMM3DPP_DIPOLE_OFF;
         cfg.divide_4pi = args_.divide_4pi;
         int rc = fmm3dpp_plan_create(&cfg, args_.device, &plan_);
@@ -130,6 +225,160 @@ inline Bodies3 generate_random_bodies(int num, real r, vec3r offset = vec3r(0, 0
     if (zero_netcharge)
         for (auto& b : bodies) b.q -= q_avg;
     return bodies;
+}
+
+inline void title(const std::string& s) { std::cout << "<" << s << ">\n\n"; }
+
+// scale_bodies (src/body.cpp:88-96): p and f divided by 4 pi.
+inline void scale_bodies(Bodies3& bs, real fac = 4 * M_PI) {
+    for (auto& b : bs) {
+        b.p /= fac;
+        for (int d = 0; d < 3; d++) b.f[d] /= fac;
+    }
+}
+
+// dipole_correction (src/kernel.cpp:1580-1596), the reference's form: p -= c |M|^2 / (n q), f -= c M.
+inline void dipole_correction(Bodies3& bs, real cycle) {
+    real M[3] = {0, 0, 0};
+    for (auto& b : bs)
+        for (int d = 0; d < 3; d++) M[d] += b.x[d] * b.q;
+    const real coef = 4 * M_PI / (3 * cycle * cycle * cycle);
+    const real dn = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
+    const real n = (real)bs.size();
+    for (auto& b : bs) {
+        b.p -= coef * dn / n / b.q;
+        for (int d = 0; d < 3; d++) b.f[d] -= coef * M[d];
+    }
+}
+
+namespace detail {
+inline void soa(const Bodies3& bs, std::vector<real>& x, std::vector<real>& y, std::vector<real>& z,
+                std::vector<real>& q) {
+    const size_t n = bs.size();
+    x.resize(n); y.resize(n); z.resize(n); q.resize(n);
+    for (size_t i = 0; i < n; i++) { x[i] = bs[i].x[0]; y[i] = bs[i].x[1]; z[i] = bs[i].x[2]; q[i] = bs[i].q; }
+}
+inline void store(Bodies3& out, const std::vector<real>& phi, const std::vector<real>& grad) {
+    for (size_t i = 0; i < out.size(); i++) {
+        out[i].p += phi[i];
+        for (int d = 0; d < 3; d++) out[i].f[d] += grad[3 * i + d];
+    }
+}
+}  // namespace detail
+
+// LaplaceKernel::direct (src/kernel.cpp:11-77) on the GPU: bs_tar.p/f += the direct lattice sum of
+// bs_src over (3^images)^3 copies; sources and targets are the same bodies, as the reference's
+// test driver calls it.
+class LaplaceKernel {
+public:
+    explicit LaplaceKernel(int device = 0) : device_(device) {}
+    void direct(const Bodies3& bs_src, Bodies3& bs_tar, int images, real cycle) {
+        if (&bs_src != &bs_tar && bs_src.size() != bs_tar.size())
+            throw std::runtime_error("direct: sources and targets must be the same bodies");
+        std::vector<real> x, y, z, q;
+        detail::soa(bs_src, x, y, z, q);
+        const int n = (int)x.size();
+        std::vector<int32_t> tar(n);
+        for (int i = 0; i < n; i++) tar[i] = i;
+        std::vector<real> phi(n), grad(3 * (size_t)n);
+        int rc = fmm3dpp_direct(device_, n, x.data(), y.data(), z.data(), q.data(), n, tar.data(), cycle, images,
+                                phi.data(), grad.data());
+        if (rc != FMM3DPP_OK) throw std::runtime_error("fmm3dpp_direct failed: " + std::to_string(rc));
+        detail::store(bs_tar, phi, grad);
+    }
+
+private:
+    int device_;
+};
+
+// EwaldSolver (include/ewald.h:20-21, src/ewald.cpp:6-203) on the GPU with the reference's
+// parameters: alpha = ksize / cycle, cutoff = cycle / 2.
+class EwaldSolver {
+public:
+    EwaldSolver(const Bodies3& bs, const Argument& args) : bs_(bs), args_(args) {}
+    Bodies3 solve() {
+        std::vector<real> x, y, z, q;
+        detail::soa(bs_, x, y, z, q);
+        const int n = (int)x.size();
+        std::vector<int32_t> tar(n);
+        for (int i = 0; i < n; i++) tar[i] = i;
+        std::vector<real> phi(n), grad(3 * (size_t)n);
+        int rc = fmm3dpp_ewald(args_.device, n, x.data(), y.data(), z.data(), q.data(), n, tar.data(), args_.cycle,
+                               args_.ewald_ksize, args_.ewald_ksize / args_.cycle, args_.cycle / 2, phi.data(),
+                               grad.data());
+        if (rc != FMM3DPP_OK) throw std::runtime_error("fmm3dpp_ewald failed: " + std::to_string(rc));
+        Bodies3 out = bs_;
+        for (auto& b : out) { b.p = 0; b.f = vec3r(0, 0, 0); }
+        detail::store(out, phi, grad);
+        return out;
+    }
+
+private:
+    Bodies3 bs_;
+    Argument args_;
+};
+
+// compare / BodyCompareResult::show (src/body.cpp:131-175, 192-205), same formulas and print format.
+struct BodyCompareResult {
+    std::string name1, name2;
+    int num_compared = 0;
+    real l2p = 0, l2f = 0, l2e = 0, rmsp = 0, rmsf = 0, epot1 = 0, epot2 = 0;
+    void show() const {
+        int bar = 72 - (int)name1.size() - (int)name2.size() - 4, left = bar / 2;
+        for (int i = 0; i < left; i++) printf("-");
+        printf("%s vs %s", name1.c_str(), name2.c_str());
+        for (int i = 0; i < bar - left; i++) printf("-");
+        printf("[%d]\n", num_compared);
+        printf("%-8s : %8.5e   %-8s : %8.5e   %-8s : %8.5e\n", "L2  (p)", l2p, "L2  (f)", l2f, "L2  (e)", l2e);
+        printf("%-8s : %8.5e   %-8s : %8.5e\n", "Rms (p)", rmsp, "Rms (f)", rmsf);
+        printf("p-energy1 : %8.12e\n", epot1);
+        printf("p-energy2 : %8.12e\n", epot2);
+        printf("\n");
+    }
+};
+
+inline BodyCompareResult compare(const Bodies3& bs1, const Bodies3& bs2, std::string name1, std::string name2,
+                                 int num_compare) {
+    if (bs1.size() != bs2.size()) throw std::runtime_error("inconsistent size in comparison");
+    BodyCompareResult res;
+    res.name1 = name1;
+    res.name2 = name2;
+    const int num = num_compare == -1 ? (int)bs1.size() : std::min(num_compare, (int)bs1.size());
+    res.num_compared = num;
+    real pdif = 0, pnrm = 0, fdif = 0, fnrm = 0, e1 = 0, e2 = 0;
+    for (int i = 0; i < num; i++) {
+        const Body3 &b1 = bs1[i], &b2 = bs2[i];
+        e1 += b1.p * b1.q;
+        e2 += b2.p * b2.q;
+        pdif += (b1.p - b2.p) * (b1.p - b2.p);
+        pnrm += b2.p * b2.p;
+        for (int d = 0; d < 3; d++) {
+            fdif += (b1.f[d] - b2.f[d]) * (b1.f[d] - b2.f[d]);
+            fnrm += b2.f[d] * b2.f[d];
+        }
+    }
+    printf("pnrm = %.8f\n", pnrm);
+    res.rmsp = std::sqrt(pdif / num);
+    res.rmsf = std::sqrt(fdif / num);
+    res.l2p = std::sqrt(pdif / pnrm);
+    res.l2f = std::sqrt(fdif / fnrm);
+    res.epot1 = e1;
+    res.epot2 = e2;
+    res.l2e = std::sqrt((e1 - e2) * (e1 - e2) / e2 / e2);
+    return res;
+}
+
+// print_bodies (src/body.cpp:16-26)
+inline void print_bodies(const Bodies3& bs, int num, int offset, const std::string& name) {
+    std::cout << name << ":" << std::endl;
+    if (num == -1) num = (int)bs.size();
+    const int s = std::min(num, (int)bs.size());
+    for (int i = offset; i < offset + s && i < (int)bs.size(); i++) {
+        const Body3& b = bs[i];   // print_body (src/body.cpp:5-14)
+        printf("[%d],(%.4f,%.4f,%.4f),%.4f,%.12f,(%.8f,%.8f,%.8f)\n", b.idx, b.x[0], b.x[1], b.x[2], b.q, b.p, b.f[0],
+               b.f[1], b.f[2]);
+    }
+    printf("\n");
 }
 
 }  // namespace rtfmm

@@ -97,4 +97,10 @@ hipError_t ewald_device(int64_t n, const double* x, const double* y, const doubl
                         const int32_t* tar, double cycle, int K, double alpha, double cutoff, double* phi,
                         double* grad, EwaldScratch& ws, hipStream_t st);
 
+// Direct lattice sum over the (3^images)^3 copies at ntar targets (device arrays), float64; part is
+// scratch for the per-chunk partial sums.
+hipError_t direct_device(int64_t n, const double* x, const double* y, const double* z, const double* q, int ntar,
+                         const int32_t* tar, double cycle, int images, double* phi, double* grad, DBuf& part,
+                         hipStream_t st);
+
 }  // namespace fmm3dpp

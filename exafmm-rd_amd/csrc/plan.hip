@@ -576,6 +576,33 @@ double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
     return -1.0;
 }
 
+int fmm3dpp_save_operators(fmm3dpp_plan* plan, const char* path) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (!path) FAIL(FMM3DPP_ERR_ARG, "null path");
+    if (plan->device >= 0) HIPCK(hipSetDevice(plan->device));
+    int rc = ensure_ops(plan);
+    if (rc) return rc;
+    return save_operators(plan->ops, plan->cfg.cycle, path, plan->err);
+}
+
+int fmm3dpp_load_operators(fmm3dpp_plan* plan, const char* path) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (!path) FAIL(FMM3DPP_ERR_ARG, "null path");
+    double t0 = now_ms();
+    Operators o;
+    int rc = load_operators(o, plan->cfg.P, plan->cfg.images, plan->cfg.cycle, path, plan->err);
+    if (rc) return rc;
+    plan->ops = std::move(o);
+    plan->ops_ready = true;
+    plan->ops_uploaded = false;
+    plan->host_ms["precompute"] = now_ms() - t0;
+    if (plan->device >= 0) {
+        HIPCK(hipSetDevice(plan->device));
+        return upload_operators(plan);
+    }
+    return FMM3DPP_OK;
+}
+
 int fmm3dpp_evaluate_adjoint_device(fmm3dpp_plan* plan, const double* d_q, const double* d_w, const double* d_grad,
                                     double* d_dq, double* d_dx) {
     if (!plan) return FMM3DPP_ERR_ARG;
@@ -625,6 +652,44 @@ int fmm3dpp_evaluate_adjoint(fmm3dpp_plan* plan, const double* q, const double* 
     HIPCK(hipMemcpyAsync(dq, plan->d_adj_dq.p, n * 8, hipMemcpyDeviceToHost, s));
     HIPCK(hipMemcpyAsync(dx, plan->d_adj_dx.p, n * 24, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_direct_device(void* hip_stream, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                          const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int images,
+                          double* d_phi, double* d_grad) {
+    if (n < 1 || ntar < 1 || !d_x || !d_y || !d_z || !d_q || !d_targets || !d_phi || !d_grad) return FMM3DPP_ERR_ARG;
+    if (images < 0 || images > 10 || !(cycle > 0)) return FMM3DPP_ERR_ARG;
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    DBuf part;
+    if (direct_device(n, d_x, d_y, d_z, d_q, ntar, d_targets, cycle, images, d_phi, d_grad, part, st) != hipSuccess)
+        return FMM3DPP_ERR_HIP;
+    return hipStreamSynchronize(st) == hipSuccess ? FMM3DPP_OK : FMM3DPP_ERR_HIP;
+}
+
+int fmm3dpp_direct(int device, int64_t n, const double* x, const double* y, const double* z, const double* q,
+                   int32_t ntar, const int32_t* targets, double cycle, int images, double* phi, double* grad) {
+    if (n < 1 || ntar < 1 || !x || !y || !z || !q || !targets || !phi || !grad) return FMM3DPP_ERR_ARG;
+    for (int32_t t = 0; t < ntar; t++)
+        if (targets[t] < 0 || targets[t] >= n) return FMM3DPP_ERR_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return FMM3DPP_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return FMM3DPP_ERR_DEVICE;
+    DBuf dx, dy, dz, dq, dt, dp, dg;
+    const size_t nb = (size_t)n * 8;
+    if (dx.ensure(nb) || dy.ensure(nb) || dz.ensure(nb) || dq.ensure(nb) || dt.ensure((size_t)ntar * 4) ||
+        dp.ensure((size_t)ntar * 8) || dg.ensure((size_t)ntar * 24))
+        return FMM3DPP_ERR_ALLOC;
+    if (hipMemcpy(dx.p, x, nb, hipMemcpyHostToDevice) || hipMemcpy(dy.p, y, nb, hipMemcpyHostToDevice) ||
+        hipMemcpy(dz.p, z, nb, hipMemcpyHostToDevice) || hipMemcpy(dq.p, q, nb, hipMemcpyHostToDevice) ||
+        hipMemcpy(dt.p, targets, (size_t)ntar * 4, hipMemcpyHostToDevice))
+        return FMM3DPP_ERR_HIP;
+    int rc = fmm3dpp_direct_device(nullptr, n, dx.as<double>(), dy.as<double>(), dz.as<double>(), dq.as<double>(),
+                                   ntar, dt.as<int32_t>(), cycle, images, dp.as<double>(), dg.as<double>());
+    if (rc) return rc;
+    if (hipMemcpy(phi, dp.p, (size_t)ntar * 8, hipMemcpyDeviceToHost) ||
+        hipMemcpy(grad, dg.p, (size_t)ntar * 24, hipMemcpyDeviceToHost))
+        return FMM3DPP_ERR_HIP;
     return FMM3DPP_OK;
 }
 

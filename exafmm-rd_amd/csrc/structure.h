@@ -57,6 +57,10 @@ int build_operators(int P, int images, double cycle, Operators& ops, std::string
 // Recompute the fused forms after a factor was replaced.
 void derive_fused(Operators& ops);
 
+// Operator cache file (every field, bit-exact); load checks the configuration and a checksum.
+int save_operators(const Operators& ops, double cycle, const std::string& path, std::string& err);
+int load_operators(Operators& ops, int P, int images, double cycle, const std::string& path, std::string& err);
+
 inline int surface_count(int P) { return 6 * (P - 1) * (P - 1) + 2; }
 
 }  // namespace fmm3dpp

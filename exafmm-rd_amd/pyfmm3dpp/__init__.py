@@ -26,6 +26,7 @@ EXPORTS = (
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
+    "fmm3dpp_save_operators", "fmm3dpp_load_operators", "fmm3dpp_direct", "fmm3dpp_direct_device",
 )
 
 
@@ -85,6 +86,12 @@ def lib():
         L.fmm3dpp_get_stage_vectors.argtypes = [vp, _dp, _dp]
         L.fmm3dpp_timing_ms.argtypes = [vp, ctypes.c_char_p]
         L.fmm3dpp_timing_ms.restype = ctypes.c_double
+        L.fmm3dpp_direct.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_int32, _ip,
+                                     ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.fmm3dpp_direct_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int32, vp, ctypes.c_double,
+                                            ctypes.c_int, vp, vp]
+        L.fmm3dpp_save_operators.argtypes = [vp, ctypes.c_char_p]
+        L.fmm3dpp_load_operators.argtypes = [vp, ctypes.c_char_p]
         L.fmm3dpp_evaluate_adjoint.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
         L.fmm3dpp_evaluate_adjoint_device.argtypes = [vp, vp, vp, vp, vp, vp]
         L.fmm3dpp_ewald.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_int32, _ip,
@@ -145,6 +152,14 @@ class LaplaceFMM:
 
     def precompute(self):
         self._check(lib().fmm3dpp_precompute(self.h))
+
+    def save_operators(self, path):
+        """Write the precomputed operators to `path` (fmm3dpp_save_operators)."""
+        self._check(lib().fmm3dpp_save_operators(self.h, os.fsencode(path)))
+
+    def load_operators(self, path):
+        """Use operators saved by a plan of the same P, images and cycle instead of the precompute."""
+        self._check(lib().fmm3dpp_load_operators(self.h, os.fsencode(path)))
 
     def set_particles(self, x, y, z):
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
@@ -264,6 +279,21 @@ def ewald(x, y, z, q, cycle, ksize, alpha=None, cutoff=None, targets=None, devic
                              alpha, cutoff, _d(phi), _d(grad))
     if rc != OK:
         raise FMMError(rc, "fmm3dpp_ewald failed")
+    return phi, grad
+
+
+def direct(x, y, z, q, images, cycle, targets=None, device=0):
+    """Direct lattice sum on the GPU (fmm3dpp_direct): (3^images)^3 copies, no dipole term.
+    Returns phi[ntar], grad[ntar, 3]."""
+    x, y, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+    n = len(x)
+    targets = np.arange(n, dtype=np.int32) if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+    phi = np.empty(len(targets))
+    grad = np.empty((len(targets), 3))
+    rc = lib().fmm3dpp_direct(device, n, _d(x), _d(y), _d(z), _d(q), len(targets), _i(targets), cycle, images,
+                              _d(phi), _d(grad))
+    if rc != OK:
+        raise FMMError(rc, "fmm3dpp_direct failed")
     return phi, grad
 
 

@@ -124,6 +124,15 @@ int fmm3dpp_set_operator(fmm3dpp_plan* plan, const char* name, const double* in,
 /* Per-cell equivalent densities and check potentials [n_cells][S] after the last evaluate. */
 int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_check);
 
+/* ---- operator cache (the reference recomputes its operators on every solve,
+ *      src/kernel.cpp:1638-1850) ----
+ * save writes the plan's operators (computing them first if needed) to `path`; load replaces the
+ * precompute with the contents of `path`, which must come from a plan with the same P, images and
+ * cycle (checked, as is a checksum; FMM3DPP_ERR_ARG otherwise).  Loaded operators are bit-identical
+ * to the saved ones. */
+int fmm3dpp_save_operators(fmm3dpp_plan* plan, const char* path);
+int fmm3dpp_load_operators(fmm3dpp_plan* plan, const char* path);
+
 /* ---- adjoint (no counterpart in the reference: the backward of phi = evaluate(q)) ----
  * For a loss L(phi) with w = dL/dphi, and grad = the forward gradient grad(phi) of the same plan and
  * particles, returns dq = dL/dq and dx = dL/dx (n x 3, input order).  The Laplace lattice kernel is
@@ -152,6 +161,17 @@ int fmm3dpp_ewald(int device, int64_t n, const double* x, const double* y, const
 int fmm3dpp_ewald_device(void* hip_stream, int64_t n, const double* d_x, const double* d_y, const double* d_z,
                          const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int ksize,
                          double alpha, double cutoff, double* d_phi, double* d_grad);
+
+/* ---- direct lattice sum (the reference's LaplaceKernel::direct, src/kernel.cpp:11-77) ----
+ * phi_t = sum over the (3^images)^3 copies of the box and all bodies of q / r (r = 0 skipped),
+ * grad likewise; no dipole term.  float64 on the GPU, cost ntar x n x 27^images.  Host arrays
+ * (fmm3dpp_direct, on `device`) or device arrays on the current device (fmm3dpp_direct_device,
+ * ordered on hip_stream, returns after the stream has finished).  images <= 10. */
+int fmm3dpp_direct(int device, int64_t n, const double* x, const double* y, const double* z, const double* q,
+                   int32_t ntar, const int32_t* targets, double cycle, int images, double* phi, double* grad);
+int fmm3dpp_direct_device(void* hip_stream, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                          const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int images,
+                          double* d_phi, double* d_grad);
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),

@@ -148,3 +148,30 @@ def test_operators_agree_with_restatement_on_physical_input(ops, oracle_mod):
     a = _K(far, _surf(P, 1.05 * r0)) @ (ops["M2M"][0] @ qc)
     b = _K(far, _surf(P, 1.05 * r0)) @ (orc.get_operator("M2M0").reshape(S, S) @ qc)
     assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(b)
+
+
+def test_operator_cache_round_trip(tmp_path):
+    """fmm3dpp_save_operators / fmm3dpp_load_operators: every operator comes back bit-identical;
+    a file for another configuration or a damaged file is refused."""
+    import pyfmm3dpp
+    a = pyfmm3dpp.LaplaceFMM(4, 16, 2, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    a.precompute()
+    path = str(tmp_path / "ops_p4_i2.bin")
+    a.save_operators(path)
+    b = pyfmm3dpp.LaplaceFMM(4, 16, 2, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    b.load_operators(path)
+    names = pyfmm3dpp.OPERATOR_NAMES + pyfmm3dpp.IMAGE_OPERATOR_NAMES + ["GK27", "GK27_SUM", "M2M_IMG_SUM"]
+    for name in names:
+        assert np.array_equal(a.get_operator(name), b.get_operator(name)), name
+    wrong = pyfmm3dpp.LaplaceFMM(4, 16, 3, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    with pytest.raises(pyfmm3dpp.FMMError):
+        wrong.load_operators(path)
+    data = bytearray(open(path, "rb").read())
+    data[len(data) // 2] ^= 0x55
+    bad = str(tmp_path / "bad.bin")
+    open(bad, "wb").write(bytes(data))
+    c = pyfmm3dpp.LaplaceFMM(4, 16, 2, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    with pytest.raises(pyfmm3dpp.FMMError):
+        c.load_operators(bad)
+    with pytest.raises(pyfmm3dpp.FMMError):
+        c.load_operators(str(tmp_path / "missing.bin"))
