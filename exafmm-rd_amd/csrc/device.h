@@ -26,6 +26,10 @@ struct BodiesDev {
     const double* q;
 };
 
+// Launchers refuse views with an unallocated array (a null device pointer would fault the kernel).
+inline bool valid(const CellsDev& c) { return c.x && c.y && c.z && c.r && c.depth && c.boff && c.bnum; }
+inline bool valid(const BodiesDev& b) { return b.x && b.y && b.z && b.q; }
+
 // Column-gathered batched GEMM, batch z = blockIdx.z:
 //   Y_z[out_idx[z*idx_zstride + c]][0:S] (+)= scale_c * A_z * X[in_idx[z*idx_zstride + c]][0:S]
 // with A_z = A + z*a_zstride (S x S row-major), X and Y_z = Y + z*y_zstride laid out [vector][S];
@@ -88,6 +92,17 @@ hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
                            const double* gy, const double* gz, const double* partial, int nblocks, int mode,
                            double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
+
+// Batches: moments and the final correction per segment (system) of the tree-ordered bodies;
+// partial holds nseg x nblocks x 5 doubles, moments nseg x 5.  A fill kernel instead of
+// hipMemsetAsync, so that the stage sequence can be captured in a graph.
+hipError_t launch_dipole_seg(BodiesDev b, const int64_t* seg_off, int nseg, int nblocks, double* partial,
+                             double* moments, hipStream_t st);
+hipError_t launch_finalize_seg(int64_t n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
+                               const double* gy, const double* gz, const int64_t* seg_off, int nseg,
+                               const double* moments, int mode, double cycle, double out_scale, double* phi,
+                               double* grad, hipStream_t st);
+hipError_t launch_fill_zero(double* a, int64_t n, hipStream_t st);
 
 // Tin-foil Ewald sum at ntar target bodies (device arrays), float64; K = max |k_i| <= 23.
 struct EwaldScratch {

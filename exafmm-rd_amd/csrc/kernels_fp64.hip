@@ -8,6 +8,8 @@
 // Each kernel cites the reference loop it replaces (paths under 3dpp/).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.h"
 
 namespace fmm3dpp {
@@ -199,16 +201,93 @@ __global__ __launch_bounds__(256) void k_adjoint_combine(int n, const int32_t* _
     dx[3 * (size_t)o + 2] = wk * grad_phi[3 * (size_t)o + 2] + qk * (grad_psi[3 * (size_t)o + 2] + sc * W * z);
 }
 
+// Segmented forms for batches of independent systems (one segment = one system's bodies in tree
+// order): moments per segment, then the correction of k_finalize with that segment's moments.
+__global__ __launch_bounds__(256) void k_dipole_seg(BodiesDev b, const int64_t* __restrict__ seg_off,
+                                                    double* __restrict__ partial) {
+    __shared__ double sm[kMoments][256];
+    const int tid = threadIdx.x, seg = blockIdx.y;
+    const int64_t i0 = seg_off[seg], i1 = seg_off[seg + 1];
+    double a[kMoments] = {0, 0, 0, 0, 0};
+    for (int64_t i = i0 + blockIdx.x * 256 + tid; i < i1; i += (int64_t)gridDim.x * 256) {
+        const double x = b.x[i], y = b.y[i], z = b.z[i], q = b.q[i];
+        a[0] += x * q;
+        a[1] += y * q;
+        a[2] += z * q;
+        a[3] += (x * x + y * y + z * z) * q;
+        a[4] += q;
+    }
+    for (int k = 0; k < kMoments; k++) sm[k][tid] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int k = 0; k < kMoments; k++) sm[k][tid] += sm[k][tid + w];
+        __syncthreads();
+    }
+    if (tid < kMoments) partial[((size_t)seg * gridDim.x + blockIdx.x) * kMoments + tid] = sm[tid][0];
+}
+
+__global__ __launch_bounds__(64) void k_moments_seg(const double* __restrict__ partial, int nblocks,
+                                                    double* __restrict__ moments) {
+    const int seg = blockIdx.x, k = threadIdx.x;
+    if (k >= kMoments) return;
+    double a = 0;
+    for (int j = 0; j < nblocks; j++) a += partial[((size_t)seg * nblocks + j) * kMoments + k];
+    moments[seg * kMoments + k] = a;
+}
+
+__global__ __launch_bounds__(256) void k_finalize_seg(int64_t n, const int32_t* __restrict__ perm, BodiesDev b,
+                                                      const double* __restrict__ p, const double* __restrict__ gx,
+                                                      const double* __restrict__ gy, const double* __restrict__ gz,
+                                                      const int64_t* __restrict__ seg_off, int nseg,
+                                                      const double* __restrict__ moments, int mode, double cycle,
+                                                      double out_scale, double* __restrict__ phi,
+                                                      double* __restrict__ grad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nseg - 1;   // segment of body i: last seg with seg_off[seg] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const double* M = moments + lo * kMoments;
+    double pv = p[i], g0 = gx[i], g1 = gy[i], g2 = gz[i];
+    if (mode != 0) {
+        const double coef = 4 * M_PI / (3 * cycle * cycle * cycle);
+        if (mode == 1) {
+            const double dn = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
+            pv -= coef * dn / (double)(seg_off[lo + 1] - seg_off[lo]) / b.q[i];
+        } else {
+            pv -= coef * (M[0] * b.x[i] + M[1] * b.y[i] + M[2] * b.z[i]) - 0.5 * coef * M[3];
+        }
+        g0 -= coef * M[0];
+        g1 -= coef * M[1];
+        g2 -= coef * M[2];
+    }
+    const int64_t o = perm[i];
+    phi[o] = pv * out_scale;
+    grad[3 * o] = g0 * out_scale;
+    grad[3 * o + 1] = g1 * out_scale;
+    grad[3 * o + 2] = g2 * out_scale;
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(double* __restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = 0.0;
+}
+
 }  // namespace
 
 hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    if (!qin || !perm || !qt) return hipErrorInvalidValue;
     k_gather<<<(n + 255) / 256, 256, 0, st>>>(qin, perm, qt, n);
     return hipGetLastError();
 }
 
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
     if (g.ncol <= 0) return hipSuccess;
+    if (!g.A || !g.X || !g.Y || !g.in_idx || !g.out_idx || g.S <= 0 || g.nz <= 0) return hipErrorInvalidValue;
     dim3 grid((g.S + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
     k_gemm_cols<<<grid, 256, 0, st>>>(g);
     return hipGetLastError();
@@ -217,12 +296,14 @@ hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
 hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
                                hipStream_t st) {
     if (ncol <= 0) return hipSuccess;
+    if (!part || !out || !Y || S <= 0) return hipErrorInvalidValue;
     int64_t tot = (int64_t)ncol * S;
     k_sum_partials<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, ncol, nz, part, out, Y);
     return hipGetLastError();
 }
 
 hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial, int nblocks, hipStream_t st) {
+    if (!partial || !valid(b) || nblocks <= 0) return hipErrorInvalidValue;
     k_dipole_partial<<<nblocks, 256, 0, st>>>(b, n, partial);
     return hipGetLastError();
 }
@@ -230,6 +311,8 @@ hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial, int nblock
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
                            const double* gy, const double* gz, const double* partial, int nblocks, int mode,
                            double cycle, double out_scale, double* phi, double* grad, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!perm || !p || !gx || !gy || !gz || !partial || !phi || !grad || !valid(b)) return hipErrorInvalidValue;
     k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, partial, nblocks, mode, cycle, out_scale,
                                                 phi, grad);
     return hipGetLastError();
@@ -239,8 +322,42 @@ hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const
                                   double c, double out_scale, const double* qin, const double* win,
                                   const double* grad_phi, const double* psi, const double* grad_psi, double* dq,
                                   double* dx, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!perm || !partial_w || !qin || !win || !grad_phi || !psi || !grad_psi || !dq || !dx || !valid(b))
+        return hipErrorInvalidValue;
     k_adjoint_combine<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, partial_w, nblocks, c, out_scale, qin, win,
                                                         grad_phi, psi, grad_psi, dq, dx);
+    return hipGetLastError();
+}
+
+hipError_t launch_dipole_seg(BodiesDev b, const int64_t* seg_off, int nseg, int nblocks, double* partial,
+                             double* moments, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    if (!seg_off || !partial || !moments || !valid(b) || nblocks <= 0) return hipErrorInvalidValue;
+    k_dipole_seg<<<dim3(nblocks, nseg), 256, 0, st>>>(b, seg_off, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_moments_seg<<<nseg, 64, 0, st>>>(partial, nblocks, moments);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_seg(int64_t n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
+                               const double* gy, const double* gz, const int64_t* seg_off, int nseg,
+                               const double* moments, int mode, double cycle, double out_scale, double* phi,
+                               double* grad, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!perm || !p || !gx || !gy || !gz || !seg_off || !moments || !phi || !grad || !valid(b) || nseg <= 0)
+        return hipErrorInvalidValue;
+    k_finalize_seg<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, perm, b, p, gx, gy, gz, seg_off, nseg, moments, mode,
+                                                               cycle, out_scale, phi, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double* a, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!a) return hipErrorInvalidValue;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    k_fill_zero<<<(unsigned)blocks, 256, 0, st>>>(a, n);
     return hipGetLastError();
 }
 

@@ -407,6 +407,7 @@ hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, con
 hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
                          const double* twiddle, const double* qeq, void* Qk, hipStream_t st) {
     if (nsrc <= 0) return hipSuccess;
+    if (!src_child || !inv_lat || !twiddle || !qeq || !Qk) return hipErrorInvalidValue;
     switch (P) {
 #define UPCASE(p)                                                                                  \
     case p:                                                                                        \
@@ -421,6 +422,7 @@ hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_ch
 hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T,
                             const void* gk8, const void* Qk, void* Pk, hipStream_t st) {
     if (ntar <= 0) return hipSuccess;
+    if (!slot_src_T || !gk8 || !Qk || !Pk || ntab < ntar) return hipErrorInvalidValue;
     return f32 ? hadamard8_t<float>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st)
                : hadamard8_t<double>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st);
 }
@@ -428,6 +430,7 @@ hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* 
 hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,
                              const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st) {
     if (nimg <= 0) return hipSuccess;
+    if (!img_tar || !img_src_T || !gk27_sum || !Qk || !Pk) return hipErrorInvalidValue;
     k_hadamard27<<<dim3((F + 255) / 256, nimg), 256, 0, st>>>(F, ntab, base, img_tar, img_src_T, gk27_sum,
                                                               reinterpret_cast<const double2*>(Qk),
                                                               reinterpret_cast<double2*>(Pk));
@@ -438,6 +441,7 @@ hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_
                            const int32_t* inv_lat, const double* twiddle, const void* Pk, double* pck,
                            hipStream_t st) {
     if (ntar <= 0) return hipSuccess;
+    if (!tar_child || !tar_scale || !inv_lat || !twiddle || !Pk || !pck) return hipErrorInvalidValue;
     switch (P) {
 #define DNCASE(p)                                                                                          \
     case p:                                                                                                \

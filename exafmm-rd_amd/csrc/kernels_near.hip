@@ -274,6 +274,7 @@ hipError_t p2m_kp(int S, const double* usurf, const int32_t* leaf_cell, int nlea
 hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                             BodiesDev b, double* pc, hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
+    if (!usurf || !leaf_cell || !pc || !valid(cells) || !valid(b)) return hipErrorInvalidValue;
     switch ((S + 63) / 64) {
         case 1: return p2m_kp<1>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
         case 2: return p2m_kp<2>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
@@ -295,6 +296,8 @@ hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
                            hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
+    if (!usurf || !leaf_cell || !qdown || !p || !gx || !gy || !gz || !valid(cells) || !valid(b))
+        return hipErrorInvalidValue;
     k_l2p_eval<<<nleaf, 64, 4 * S * sizeof(double), st>>>(S, usurf, leaf_cell, cells, b, qdown, p, gx, gy, gz);
     return hipGetLastError();
 }
@@ -303,6 +306,8 @@ hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, Bodie
                       const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
                       hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
+    if (!leaf_cell || !list_off || !list_ent || !p || !gx || !gy || !gz || !valid(cells) || !valid(b))
+        return hipErrorInvalidValue;
     k_p2p_f64<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_ent, cycle, p, gx, gy, gz);
     return hipGetLastError();
 }
@@ -310,6 +315,7 @@ hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, Bodie
 hipError_t launch_pack_rel(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, float4* rel,
                            hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
+    if (!leaf_cell || !rel || !valid(cells) || !valid(b)) return hipErrorInvalidValue;
     k_pack_rel<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, rel);
     return hipGetLastError();
 }
@@ -318,6 +324,8 @@ hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, c
                           const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
                           double* gy, double* gz, hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
+    if (!leaf_cell || !rel || !list_off || !list_ent || !p || !gx || !gy || !gz || !valid(cells))
+        return hipErrorInvalidValue;
     k_p2p_f32<<<nleaf, 64, 0, st>>>(leaf_cell, cells, rel, list_off, list_ent, cycle, p, gx, gy, gz);
     return hipGetLastError();
 }
