@@ -10,12 +10,19 @@ M2M, M2L, L2L, L2P, P2P, dipole correction) with positions and charges already i
 operator precompute depends only on (P, images) and is excluded (reported separately).  Both
 precision paths are timed the same way on the same inputs; `value` is the mixed path's throughput
 (`--path fp64` reports the float64 path instead), and the line carries the mixed-vs-float64
-difference of potential and gradient.  Prints one JSON line.
+difference of potential and gradient.
+
+The `batch` object times many uneven systems (48 by default, N log-uniform in [4096, 262144], P 6 or
+8, images 2, mixed precision): one batch (every FMM stage one launch per P group, replayed from a
+HIP graph) against one plan per system evaluated one after another, particles set once, `steps`
+evaluations each.  Prints one JSON line.
 """
 import argparse
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -59,6 +66,88 @@ def cpu_baseline(x, y, z, q, P, ncrit, images, threads, budget_s):
     return {"value": len(x) / dt, "unit": "particles/s", "cores": threads, "kind": "port",
             "sample": f"one full step (tree+lists+all stages, precompute excluded) of the same N={len(x)} workload, "
                       f"{dt:.1f} s", "seconds": round(dt, 3)}
+
+
+def batch_workload(nsys, nmin, nmax, seed):
+    """nsys systems, N log-uniform in [nmin, nmax], P alternating 6 / 8, zero-sum charges."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys)).astype(np.int64)
+    Ps = rng.choice([6, 8], nsys)
+    xs, qs = [], []
+    for n in counts:
+        pts = rng.random((int(n), 3)) - 0.5
+        q = rng.uniform(-1.0, 1.0, int(n))
+        q -= q.mean()
+        xs.append(pts)
+        qs.append(q)
+    return counts, Ps, np.concatenate(xs), np.concatenate(qs)
+
+
+def run_batch(pyfmm3dpp, torch, args):
+    """Many uneven systems: one Batch (merged launches, HIP graph) against one plan per system run
+    one after another; both time `steps` evaluations with the particles set once."""
+    from pyfmm3dpp.batch import Batch
+    counts, Ps, pts, q = batch_workload(args.batch_systems, 4096, 262144, args.seed + 7)
+    n = int(counts.sum())
+    dev = torch.device("cuda", 0)
+    d_x, d_y, d_z = (torch.from_numpy(pts[:, k].copy()).to(dev) for k in range(3))
+    d_q = torch.from_numpy(q).to(dev)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    prec = pyfmm3dpp.PREC_MIXED
+    # sequential: one plan per system
+    plans, cache = [], {}
+    tmp = tempfile.mkdtemp(prefix="fmm3dpp_ops_")
+    for k in range(len(counts)):
+        P = int(Ps[k])
+        p = pyfmm3dpp.LaplaceFMM(P, args.ncrit, 2, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_PHYSICAL,
+                                 precision=prec, device=0)
+        if P in cache:               # operators depend on (P, images, cycle) only
+            p.load_operators(cache[P])
+        else:
+            p.precompute()
+            cache[P] = os.path.join(tmp, f"ops_P{P}.bin")
+            p.save_operators(cache[P])
+        s = slice(int(off[k]), int(off[k + 1]))
+        p.set_particles_device(int(counts[k]), d_x[s].data_ptr(), d_y[s].data_ptr(), d_z[s].data_ptr())
+        plans.append((p, s))
+    phi_s = torch.empty(n, dtype=torch.float64, device=dev)
+    grad_s = torch.empty((n, 3), dtype=torch.float64, device=dev)
+
+    def seq_step():
+        for p, s in plans:
+            p.evaluate_device(d_q[s].data_ptr(), phi_s[s].data_ptr(), grad_s[s].data_ptr())
+
+    b = Batch([(int(P), args.ncrit) for P in Ps], 2, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_PHYSICAL, precision=prec)
+    t0 = time.perf_counter()
+    b.set_particles(counts, d_x, d_y, d_z)
+    set_s = time.perf_counter() - t0
+    b.set_graph(True)
+    phi_b = torch.empty(n, dtype=torch.float64, device=dev)
+    grad_b = torch.empty((n, 3), dtype=torch.float64, device=dev)
+
+    def batch_step():
+        b.evaluate(d_q, phi_b, grad_b)
+
+    out = {}
+    for name, step in (("sequential", seq_step), ("batch", batch_step)):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / args.steps
+    diff = max(rel_l2(torch, phi_b, phi_s), rel_l2(torch, grad_b, grad_s))
+    shutil.rmtree(tmp, ignore_errors=True)
+    return {"systems": int(len(counts)), "bodies": n, "P": sorted(set(int(p) for p in Ps)),
+            "sequential_particles_per_s": round(n / out["sequential"], 1),
+            "batch_particles_per_s": round(n / out["batch"], 1),
+            "speedup": round(out["sequential"] / out["batch"], 3),
+            "batch_ms_per_step": round(out["batch"] * 1e3, 3),
+            "sequential_ms_per_step": round(out["sequential"] * 1e3, 3),
+            "batch_set_particles_s": round(set_s, 3),
+            "rel_l2_batch_vs_sequential": float("%.3e" % diff)}
 
 
 STAGES = ["P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize", "evaluate"]
@@ -126,6 +215,9 @@ def main():
                     help="time only the --path path, no accuracy legs (for kernel profiles of one path)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-systems", type=int, default=48,
+                    help="systems of the batch leg (N log-uniform 4096..262144, P 6/8, images 2, mixed)")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the evaluation path is single-device (asked for {args.gpus}).",
@@ -222,6 +314,8 @@ def main():
         "precompute_s": round(main_run["precompute_s"], 3),
         "roofline": roofline,
     }
+    if not args.no_batch:
+        out["batch"] = run_batch(pyfmm3dpp, torch, args)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(x, y, z, q, args.P, args.ncrit, args.images, threads, 30)

@@ -17,6 +17,7 @@
 
 #include "dbuf.h"
 #include "device.h"
+#include "device_ops.h"
 #include "fmm3dpp.h"
 #include "gpu_structure.h"
 #include "structure.h"
@@ -46,8 +47,7 @@ struct fmm3dpp_plan {
     DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
     DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
     DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
-    DBuf o_ut, o_vs, o_utl, o_vsl, o_m2m_all, o_l2l, o_m2m_img, o_l2l_img, o_gk8, o_gk8f, o_gk27, o_usurf, o_invlat,
-        o_tw;
+    DeviceOperators dop;
 
     bool mixed() const { return cfg.precision == FMM3DPP_PREC_MIXED; }
 
@@ -69,13 +69,9 @@ struct fmm3dpp_plan {
         if (e_ != hipSuccess) FAIL(FMM3DPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-namespace {
+namespace fmm3dpp {
 
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-int validate(const fmm3dpp_config* c, std::string& err) {
+int validate_config(const fmm3dpp_config* c, std::string& err) {
     if (!c) { err = "null config"; return FMM3DPP_ERR_ARG; }
     if (c->P < 2 || c->P > 12) { err = "P must be in [2, 12]"; return FMM3DPP_ERR_ARG; }
     if (c->ncrit < 1) { err = "ncrit must be >= 1"; return FMM3DPP_ERR_ARG; }
@@ -89,37 +85,17 @@ int validate(const fmm3dpp_config* c, std::string& err) {
     return FMM3DPP_OK;
 }
 
+}  // namespace fmm3dpp
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 int upload_operators(fmm3dpp_plan* plan) {
-    hipStream_t s = plan->stream;
-    const Operators& o = plan->ops;
-    std::vector<double> l2l_all, m2m_all;
-    for (int k = 0; k < 8; k++) l2l_all.insert(l2l_all.end(), o.l2l[k].begin(), o.l2l[k].end());
-    for (int k = 0; k < 8; k++) m2m_all.insert(m2m_all.end(), o.m2m[k].begin(), o.m2m[k].end());
-    std::vector<double> zeros((size_t)o.S * o.S, 0.0);
-    HIPCK(upload(plan->o_ut, o.UT, s));
-    HIPCK(upload(plan->o_vs, o.VSinv, s));
-    HIPCK(upload(plan->o_utl, o.UT_l2p, s));
-    HIPCK(upload(plan->o_vsl, o.VSinv_l2p, s));
-    HIPCK(upload(plan->o_m2m_all, m2m_all, s));
-    HIPCK(upload(plan->o_l2l, l2l_all, s));
-    HIPCK(upload(plan->o_m2m_img, o.images > 0 ? o.m2m_img_sum : zeros, s));
-    HIPCK(upload(plan->o_l2l_img, o.images > 0 ? o.l2l_img : zeros, s));
-    HIPCK(upload(plan->o_gk8, o.gk8, s));
-    if (plan->mixed()) {
-        std::vector<float> g32(o.gk8.begin(), o.gk8.end());
-        HIPCK(upload(plan->o_gk8f, g32, s));
-    }
-    HIPCK(upload(plan->o_gk27, o.gk27_sum, s));
-    HIPCK(upload(plan->o_usurf, o.unit_surf, s));
-    {   // (k * P + j) * P + i of the P^3 grid corner -> surface point (surface.h:28-48), -1 inside
-        const int P = o.P;
-        std::vector<int32_t> inv((size_t)P * P * P, -1);
-        for (int sp = 0; sp < o.S; sp++)
-            inv[((size_t)o.lattice[3 * sp + 2] * P + o.lattice[3 * sp + 1]) * P + o.lattice[3 * sp]] = sp;
-        HIPCK(upload(plan->o_invlat, inv, s));
-    }
-    HIPCK(upload(plan->o_tw, o.twiddle, s));
-    HIPCK(hipStreamSynchronize(s));
+    HIPCK(plan->dop.upload(plan->ops, plan->mixed(), plan->stream));
+    HIPCK(hipStreamSynchronize(plan->stream));
     plan->ops_uploaded = true;
     return FMM3DPP_OK;
 }
@@ -190,7 +166,7 @@ int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** ou
     if (!out) return FMM3DPP_ERR_ARG;
     *out = nullptr;
     std::string err;
-    int rc = validate(cfg, err);
+    int rc = validate_config(cfg, err);
     if (rc) return rc;
     std::unique_ptr<fmm3dpp_plan> plan(new fmm3dpp_plan());
     plan->cfg = *cfg;
@@ -305,27 +281,27 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     HIPCK(launch_gather(d_qin, g.perm.as<int32_t>(), plan->d_q.as<double>(), n, s));
     HIPCK(hipMemsetAsync(pck, 0, nc * Sn * sizeof(double), s));
     // P2M (fmm.cpp:83-109): check potentials, then q_equiv = 0.5^depth * VSinv * (UT * p_check)
-    HIPCK(launch_p2m_check(Sn, plan->o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
+    HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
                            plan->d_pcleaf.as<double>(), s));
     {
-        GemmCols g1{Sn, nleaf, 1, 0, plan->o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
+        GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
                     plan->d_pcleaf2.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, plan->o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota,
+        GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota,
                     g.leaf_cell.as<int32_t>(), 0, qeq, 0, g.leaf_depth.as<int32_t>()};
         HIPCK(launch_gemm_cols(g2, s));
     }
     HIPCK(hipEventRecord(plan->ev[1], s));
     // M2M (fmm.cpp:111-150): per level, one GEMM per child octant into partials, then the child sum
     for (const auto& lv : g.m2m_levels) {
-        GemmCols gm{Sn, lv.ncol, 8, 0, plan->o_m2m_all.as<double>(), (int64_t)SS, qeq,
+        GemmCols gm{Sn, lv.ncol, 8, 0, plan->dop.o_m2m_all.as<double>(), (int64_t)SS, qeq,
                     g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol, plan->d_m2m_part.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(gm, s));
         HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
                                   qeq, s));
     }
     for (size_t i = 0; i < g.img_m2m.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 0, plan->o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
+        GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
                     g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
         HIPCK(launch_gemm_cols(gi, s));
     }
@@ -334,29 +310,29 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     // Targets = sources = the tree's non-leaf cells [0, nl), then the image cells [nl, ntar).
     const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;
     const int32_t* slots = g.child_slots.as<int32_t>();
-    const int32_t* inv = plan->o_invlat.as<int32_t>();
-    const double* tw = plan->o_tw.as<double>();
+    const int32_t* inv = plan->dop.o_invlat.as<int32_t>();
+    const double* tw = plan->dop.o_tw.as<double>();
     if (!plan->mixed()) {
         HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8.p, plan->d_Qk.p,
+        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->dop.o_gk8.p, plan->d_Qk.p,
                                plan->d_Pk.p, s));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
-                                plan->o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
+                                plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
         HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
     } else {
         // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
         HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->o_gk8f.p, plan->d_Qk.p,
+        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->dop.o_gk8f.p, plan->d_Qk.p,
                                plan->d_Pk.p, s));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
         if (nimg > 0) {
             HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
             HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, nl,
-                                    plan->o_gk27.as<double>(), plan->d_Qk_img.as<double>(),
+                                    plan->dop.o_gk27.as<double>(), plan->d_Qk_img.as<double>(),
                                     plan->d_Pk_img.as<double>(), s));
             HIPCK(launch_m2l_down(o.P, false, Sn, nimg, slots + (size_t)nl * 8, g.tar_scale.as<double>() + nl, inv,
                                   tw, plan->d_Pk_img.p, pck, s));
@@ -366,26 +342,26 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     // L2L (fmm.cpp:195-229): image levels from the outermost image, then one launch per tree level
     const size_t nimgm2m = g.img_m2m.size();
     for (size_t i = 0; i < g.img_l2l.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 1, plan->o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + nimgm2m + i,
+        GemmCols gi{Sn, 1, 1, 1, plan->dop.o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + nimgm2m + i,
                     g.img_b.as<int32_t>() + nimgm2m + i, 0, pck, 0, nullptr};
         HIPCK(launch_gemm_cols(gi, s));
     }
     for (const auto& lv : g.l2l_levels) {
-        GemmCols gl{Sn, lv.ncol, 8, 1, plan->o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
+        GemmCols gl{Sn, lv.ncol, 8, 1, plan->dop.o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
                     g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
         HIPCK(launch_gemm_cols(gl, s));
     }
     HIPCK(hipEventRecord(plan->ev[6], s));
     // L2P (fmm.cpp:231-257): q_down = 0.5^depth * VSinv_l2p * (UT_l2p * p_check), then evaluation
     {
-        GemmCols g1{Sn, nleaf, 1, 0, plan->o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
+        GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
                     plan->d_pcleaf2.as<double>(), 0, nullptr};
         HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, plan->o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
+        GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
                     plan->d_pcleaf.as<double>(), 0, g.leaf_depth.as<int32_t>()};
         HIPCK(launch_gemm_cols(g2, s));
     }
-    HIPCK(launch_l2p_eval(Sn, plan->o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
+    HIPCK(launch_l2p_eval(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
                           plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
                           plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
     HIPCK(hipEventRecord(plan->ev[7], s));

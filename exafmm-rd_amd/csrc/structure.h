@@ -63,4 +63,7 @@ int load_operators(Operators& ops, int P, int images, double cycle, const std::s
 
 inline int surface_count(int P) { return 6 * (P - 1) * (P - 1) + 2; }
 
+// Checks a configuration (ranges of P, ncrit, images, cycle, centre, precision, dipole mode).
+int validate_config(const fmm3dpp_config* c, std::string& err);
+
 }  // namespace fmm3dpp

@@ -27,6 +27,9 @@ EXPORTS = (
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
     "fmm3dpp_save_operators", "fmm3dpp_load_operators", "fmm3dpp_direct", "fmm3dpp_direct_device",
+    "fmm3dpp_batch_create", "fmm3dpp_batch_destroy", "fmm3dpp_batch_last_error",
+    "fmm3dpp_batch_set_particles_device", "fmm3dpp_batch_evaluate_device", "fmm3dpp_batch_set_graph",
+    "fmm3dpp_batch_autotune", "fmm3dpp_batch_timing_ms",
 )
 
 
@@ -98,6 +101,16 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.fmm3dpp_ewald_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int32, vp, ctypes.c_double,
                                            ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp]
+        L.fmm3dpp_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int32, ctypes.c_int, ctypes.POINTER(vp)]
+        L.fmm3dpp_batch_destroy.argtypes = [vp]
+        L.fmm3dpp_batch_last_error.argtypes = [vp]
+        L.fmm3dpp_batch_last_error.restype = ctypes.c_char_p
+        L.fmm3dpp_batch_set_particles_device.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), vp, vp, vp]
+        L.fmm3dpp_batch_evaluate_device.argtypes = [vp, vp, vp, vp]
+        L.fmm3dpp_batch_set_graph.argtypes = [vp, ctypes.c_int]
+        L.fmm3dpp_batch_autotune.argtypes = [vp, _ip, ctypes.c_int32, vp, vp, vp, _ip]
+        L.fmm3dpp_batch_timing_ms.argtypes = [vp, ctypes.c_char_p]
+        L.fmm3dpp_batch_timing_ms.restype = ctypes.c_double
         _lib = L
     return _lib
 

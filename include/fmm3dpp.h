@@ -173,6 +173,33 @@ int fmm3dpp_direct_device(void* hip_stream, int64_t n, const double* d_x, const 
                           const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int images,
                           double* d_phi, double* d_grad);
 
+/* ---- batches of independent periodic systems (no counterpart in the reference, whose LaplaceFMM
+ *      solves one system; for callers with many systems of uneven size) ----
+ * nsys configurations: images, cycle, precision, dipole_mode and divide_4pi must agree; P, ncrit and
+ * center may differ.  Each system gets its own tree and lists; the systems of equal P are merged,
+ * so every FMM stage is one launch over all of them.  Results equal those of one plan per system.
+ * Positions, charges, potentials and gradients of all systems are concatenated in system order
+ * (counts[k] bodies for system k), device arrays on the batch's device; calls return after the
+ * batch's stream has finished. */
+typedef struct fmm3dpp_batch fmm3dpp_batch;
+int fmm3dpp_batch_create(const fmm3dpp_config* cfgs, int32_t nsys, int device, fmm3dpp_batch** out);
+int fmm3dpp_batch_destroy(fmm3dpp_batch* batch);
+const char* fmm3dpp_batch_last_error(const fmm3dpp_batch* batch);
+int fmm3dpp_batch_set_particles_device(fmm3dpp_batch* batch, const int64_t* counts, const double* d_x,
+                                       const double* d_y, const double* d_z);
+int fmm3dpp_batch_evaluate_device(fmm3dpp_batch* batch, const double* d_q, double* d_phi, double* d_grad);
+/* enable != 0: the evaluation's launches are captured once into a HIP graph (per set of d_q, d_phi,
+ * d_grad pointers) and replayed; set_particles and autotune drop the graph. */
+int fmm3dpp_batch_set_graph(fmm3dpp_batch* batch, int enable);
+/* Leaf size per P group: for each candidate ncrit, rebuilds the group's trees and lists and times one
+ * evaluation with charges d_q (outputs land in d_phi, d_grad); keeps the fastest (tree build +
+ * evaluation).  chosen[nsys] (may be NULL) receives the ncrit of every system. */
+int fmm3dpp_batch_autotune(fmm3dpp_batch* batch, const int32_t* candidates, int32_t ncand, const double* d_q,
+                           double* d_phi, double* d_grad, int32_t* chosen);
+/* Milliseconds of the last evaluate ("evaluate", HIP events), set_particles ("set_particles", host
+ * wall) and operator precompute ("precompute"); -1 if unknown. */
+double fmm3dpp_batch_timing_ms(const fmm3dpp_batch* batch, const char* name);
+
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
  * L2L, L2P, P2P, finalize, evaluate; host wall times: tree_and_lists (last set_particles),

@@ -125,3 +125,22 @@ def test_operator_queries_on_host_plan():
     m2m = plan.get_operator("M2M3").reshape(S, S)
     assert np.array_equal(plan.get_operator("L2L3").reshape(S, S), m2m.T)
     assert np.array_equal(plan.get_operator("L2L_IMG").reshape(S, S), plan.get_operator("M2M_IMG13").reshape(S, S).T)
+
+
+def test_batch_config_checks():
+    """Batch creation refuses invalid or disagreeing configurations before it looks for a device."""
+    import pyfmm3dpp
+    L = pyfmm3dpp.lib()
+    h = ctypes.c_void_p()
+    assert L.fmm3dpp_batch_create(None, 1, 0, ctypes.byref(h)) == pyfmm3dpp.ERR_ARG
+    for bad in (dict(images=2), dict(cycle=2.0), dict(precision=1), dict(dipole_mode=2), dict(divide_4pi=1),
+                dict(P=13)):
+        cfgs = (pyfmm3dpp.Config * 2)(_cfg(), _cfg(**bad))
+        assert L.fmm3dpp_batch_create(cfgs, 2, 0, ctypes.byref(h)) == pyfmm3dpp.ERR_ARG and not h.value
+    cfgs = (pyfmm3dpp.Config * 2)(_cfg(), _cfg(P=6, ncrit=64))   # P and ncrit may differ
+    rc = L.fmm3dpp_batch_create(cfgs, 2, 0, ctypes.byref(h))
+    assert rc in (pyfmm3dpp.OK, pyfmm3dpp.ERR_DEVICE)
+    if rc == pyfmm3dpp.OK:
+        assert L.fmm3dpp_batch_destroy(h) == pyfmm3dpp.OK
+    assert L.fmm3dpp_batch_destroy(None) == pyfmm3dpp.ERR_ARG
+    assert L.fmm3dpp_batch_timing_ms(None, b"evaluate") == -1

@@ -53,5 +53,5 @@ def test_reference_log_blocks_reproduced():
     e2 = float(re.search(r"p-energy2 : (\S+)", block).group(1))
     assert abs(e1 - row["energy_fmm"]) <= 2e-10 * abs(row["energy_fmm"])
     assert abs(e2 - row["energy_ref"]) <= 1e-9 * abs(row["energy_ref"])
-    l2f = float(re.search(r"L2  \(f\) : (\S+)", block).group(1))
+    l2f = float(re.search(r"L2  \(f\)  : (\S+)", block).group(1))
     assert abs(l2f - row["l2f"]) <= 1e-4 * row["l2f"]
