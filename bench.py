@@ -218,6 +218,9 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=48,
                     help="systems of the batch leg (N log-uniform 4096..262144, P 6/8, images 2, mixed)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain copy kernel of 1 GiB read + 1 GiB written (elementwise x * 1.0), "
+                         "the reference point for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the evaluation path is single-device (asked for {args.gpus}).",
@@ -229,6 +232,12 @@ def main():
 
     x, y, z, q = workload(args.n, args.seed)
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     d_x, d_y, d_z = (torch.from_numpy(a).to(dev) for a in (x, y, z))
     d_q = torch.from_numpy(q).to(dev)
     torch.cuda.synchronize()

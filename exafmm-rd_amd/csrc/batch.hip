@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -15,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dbuf.h"
@@ -31,6 +33,8 @@ namespace {
 // Blocks per system of the dipole-moment reduction: the same partition and summation order as the
 // single-system plan, so a system's moments are bit-identical to its own plan's.
 constexpr int kDipoleBlocks = 256;
+// Host threads (and streams) for the per-system tree builds.
+constexpr int kBuildThreads = 8;
 
 // One merged group: the systems with the same P.
 struct Group {
@@ -58,6 +62,7 @@ struct fmm3dpp_batch {
     int device = 0;
     std::string err;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> build_streams;   // one per build thread
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<std::unique_ptr<GpuStructure>> gs;
     std::vector<int64_t> counts, offsets;   // bodies per system and their offsets in the caller's arrays
@@ -81,8 +86,9 @@ struct fmm3dpp_batch {
         drop_graph();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        groups.clear();   // device buffers, freed before the stream goes away
+        groups.clear();   // device buffers, freed before the streams go away
         gs.clear();
+        for (hipStream_t bs : build_streams) (void)hipStreamDestroy(bs);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -152,6 +158,33 @@ hipError_t copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
 }
 
+// The single-system GPU builds of systems `ids`, spread over the batch's build threads, each with
+// its own stream (a build synchronises its stream at every tree level and frontier step, so several
+// in flight keep the GPU busy).  Returns after every build has finished.
+int build_systems(fmm3dpp_batch* b, const std::vector<int>& ids) {
+    BHIP(hipStreamSynchronize(b->stream));   // positions copied on the batch stream
+    const int nthr = std::min<int>((int)b->build_streams.size(), (int)ids.size());
+    std::atomic<int> next{0};
+    std::vector<int> rcs(nthr, FMM3DPP_OK);
+    std::vector<std::string> errs(nthr);
+    auto work = [&](int w) {
+        if (hipSetDevice(b->device) != hipSuccess) { rcs[w] = FMM3DPP_ERR_DEVICE; errs[w] = "hipSetDevice failed"; return; }
+        for (int i = next++; i < (int)ids.size(); i = next++) {
+            const int k = ids[i];
+            const int rc = b->gs[k]->build(b->cfg[k], b->counts[k], b->px + b->offsets[k], b->py + b->offsets[k],
+                                           b->pz + b->offsets[k], b->build_streams[w], errs[w]);
+            if (rc) { rcs[w] = rc; errs[w] = "system " + std::to_string(k) + ": " + errs[w]; return; }
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int w = 1; w < nthr; w++) threads.emplace_back(work, w);
+    work(0);
+    for (auto& t : threads) t.join();
+    for (int w = 0; w < nthr; w++)
+        if (rcs[w]) BFAIL(rcs[w], errs[w]);
+    return FMM3DPP_OK;
+}
+
 // Builds the trees of the group's systems (one GPU build each) and merges their tables on the GPU:
 // cells, leaves and bodies are concatenated in system order; the M2L targets are the regular targets
 // of every system, then the image targets of every system; the operator-apply tables of tree level l
@@ -159,11 +192,8 @@ hipError_t copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
 int build_group(fmm3dpp_batch* b, Group& g) {
     hipStream_t s = b->stream;
     const int images = b->cfg[0].images;
-    for (int k : g.sys) {
-        int rc = b->gs[k]->build(b->cfg[k], b->counts[k], b->px + b->offsets[k], b->py + b->offsets[k],
-                                 b->pz + b->offsets[k], s, b->err);
-        if (rc) return rc;
-    }
+    int rc = build_systems(b, g.sys);
+    if (rc) return rc;
     // merged sizes and offsets (host mirrors of the per-system sizes)
     int64_t nb = 0, nent = 0, ncell64 = 0;
     int nleaf = 0, maxd = 0, nl = 0, nimg = 0;
@@ -466,6 +496,11 @@ int fmm3dpp_batch_create(const fmm3dpp_config* cfgs, int32_t nsys, int device, f
     b->device = device;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
     if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return FMM3DPP_ERR_HIP;
+    for (int w = 0; w < std::min(nsys, kBuildThreads); w++) {
+        hipStream_t bs = nullptr;
+        if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
+        b->build_streams.push_back(bs);
+    }
     for (int k = 0; k < nsys; k++) b->gs.emplace_back(new GpuStructure());
     std::map<int, int> by_p;
     for (int k = 0; k < nsys; k++) {

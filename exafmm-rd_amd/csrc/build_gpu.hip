@@ -40,6 +40,39 @@ constexpr int kD = 21;
 
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
 
+// Body order inside each leaf = input order (the reference's stable partition keeps it): sort each
+// leaf's slice of the key-sorted permutation ascending.  One workgroup per cell (non-leaves exit),
+// bitonic sort in LDS; leaves hold <= ncrit <= kLeafSortMax bodies.
+constexpr int kLeafSortMax = 1024;
+__global__ __launch_bounds__(256) void k_leaf_sort(int ntree, const int32_t* __restrict__ cnum,
+                                                   const int32_t* __restrict__ boff, const int32_t* __restrict__ bnum,
+                                                   const int32_t* __restrict__ perm1, int32_t* __restrict__ perm) {
+    __shared__ int32_t v[kLeafSortMax];
+    const int c = blockIdx.x;
+    if (cnum[c] != 0) return;                  // uniform over the workgroup
+    const int b = boff[c], m = bnum[c];
+    if (m > kLeafSortMax) return;              // leaves hold <= ncrit <= kLeafSortMax bodies (checked on the host)
+    int w = 1;
+    while (w < m) w <<= 1;
+    for (int i = threadIdx.x; i < w; i += blockDim.x) v[i] = i < m ? perm1[b + i] : INT32_MAX;
+    __syncthreads();
+    for (int k = 2; k <= w; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < w; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = v[i], bb = v[l];
+                    if ((a > bb) == ((i & k) == 0)) {
+                        v[i] = bb;
+                        v[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) perm[b + i] = v[i];
+}
+
 __global__ void k_keys(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, double c0x, double c0y, double c0z, double r0,
                        uint64_t* __restrict__ key, int32_t* __restrict__ idx, int* __restrict__ flag) {
@@ -520,10 +553,19 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
                                                  depth.as<int32_t>(), perm1.as<int32_t>(), ldep.as<int8_t>(),
                                                  head.as<int32_t>());
     CK(hipGetLastError());
-    CK(tkey.ensure(n * 8)); CK(tkey_s.ensure(n * 8)); CK(perm.ensure(n * 4));
-    k_tkey<<<blocks(n), 256, 0, st>>>(n, key.as<uint64_t>(), ldep.as<int8_t>(), tkey.as<uint64_t>(), idx0.as<int32_t>());
-    CK(hipGetLastError());
-    CK(sort_pairs(temp, tkey.as<uint64_t>(), tkey_s.as<uint64_t>(), idx0.as<int32_t>(), perm.as<int32_t>(), n, 3 * kD, st));
+    CK(perm.ensure(n * 4));
+    if (cfg.ncrit <= kLeafSortMax) {   // leaves hold <= ncrit bodies: sort each leaf's slice by input index
+        k_leaf_sort<<<ntree, 256, 0, st>>>(ntree, cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
+                                           perm1.as<int32_t>(), perm.as<int32_t>());
+        CK(hipGetLastError());
+    } else {                           // stable re-sort by the key truncated at the leaf depth
+        CK(tkey.ensure(n * 8)); CK(tkey_s.ensure(n * 8));
+        k_tkey<<<blocks(n), 256, 0, st>>>(n, key.as<uint64_t>(), ldep.as<int8_t>(), tkey.as<uint64_t>(),
+                                          idx0.as<int32_t>());
+        CK(hipGetLastError());
+        CK(sort_pairs(temp, tkey.as<uint64_t>(), tkey_s.as<uint64_t>(), idx0.as<int32_t>(), perm.as<int32_t>(), n,
+                      3 * kD, st));
+    }
     CK(x.ensure(n * 8)); CK(y.ensure(n * 8)); CK(z.ensure(n * 8));
     k_gather_pos<<<blocks(n), 256, 0, st>>>(n, perm.as<int32_t>(), dx, dy, dz, x.as<double>(), y.as<double>(), z.as<double>());
     CK(hipGetLastError());

@@ -25,13 +25,20 @@ __global__ void k_gather(const double* __restrict__ qin, const int32_t* __restri
 
 // Column-gathered batched GEMM for the operator applies: P2M/L2P pseudo-inverse factors
 // (src/kernel.cpp:369-370, 1465-1467), M2M per child octant (395-404), L2L per child octant
-// (1419-1428) and the image-level M2M/L2L (434-442, 1430-1435).  64x64 output tile per workgroup,
-// 4x4 per thread, K in steps of 16 with the next tile prefetched into registers.
+// (1419-1428) and the image-level M2M/L2L (434-442, 1430-1435), on the float64 matrix cores.
+// Computed transposed, D[v][r] = sum_k X[v][k] A[r][k] (v = gathered vector, r = output
+// component), so that a lane's accumulator rows are vectors and its columns consecutive
+// components: stores of 16 lanes cover 128 contiguous bytes of one output vector.
+// Workgroup tile: 64 vectors x 64 components; wave tile 32 x 32 = 2 x 2 blocks of
+// v_mfma_f64_16x16x4f64 (lane l: A operand [l % 16][l / 16], B operand [l / 16][l % 16],
+// accumulator register i = D[l / 16 + 4 i][l % 16]).  K in LDS stages of 16 with the next stage
+// prefetched into registers.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int TM = 64, TN = 64, TK = 16;
 __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
-    __shared__ double As[TK][TM + 2];
-    __shared__ double Bs[TK][TN + 2];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    __shared__ double As[TK][TM + 4];   // operator rows r (components), k-major
+    __shared__ double Bs[TK][TN + 4];   // gathered vectors v, k-major
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row0 = blockIdx.x * TM, col0 = blockIdx.y * TN, z = blockIdx.z;
     const int S = g.S, K = g.S;
     const double* A = g.A + (int64_t)z * g.a_zstride;
@@ -52,11 +59,13 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
             rb[u] = (bcol && gk < K) ? bcol[gk] : 0.0;
         }
     };
-    double acc[4][4];
+    const int n = lane & 15, kq = lane >> 4;
+    const int vb0 = 2 * (wave & 1), rb0 = 2 * (wave >> 1);   // this wave's 16-blocks of v and r
+    f64x4 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+        for (int j = 0; j < 2; j++) acc[i][j] = f64x4{0, 0, 0, 0};
     fetch(0);
     for (int k0 = 0; k0 < K; k0 += TK) {
 #pragma unroll
@@ -64,35 +73,39 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
         __syncthreads();
         if (k0 + TK < K) fetch(k0 + TK);
 #pragma unroll
-        for (int kk = 0; kk < TK; kk++) {
-            double a[4], bb[4];
+        for (int k4 = 0; k4 < TK / 4; k4++) {
+            const int kk = 4 * k4 + kq;
+            double xv[2], av[2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[kk][ty * 4 + i];
+            for (int i = 0; i < 2; i++) xv[i] = Bs[kk][(vb0 + i) * 16 + n];   // A operand: X[v][k]
 #pragma unroll
-            for (int j = 0; j < 4; j++) bb[j] = Bs[kk][tx * 4 + j];
+            for (int j = 0; j < 2; j++) av[j] = As[kk][(rb0 + j) * 16 + n];   // B operand: A[r][k]
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i], av[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int c = col0 + tx * 4 + j;
-        if (c >= g.ncol) continue;
-        int oi = out_idx[c];
-        if (oi < 0) continue;
-        double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int r = row0 + ty * 4 + i;
-            if (r >= S) continue;
-            double v = sc * acc[i][j];
-            double* y = &Y[(int64_t)oi * S + r];
-            *y = g.accumulate ? *y + v : v;
+        for (int q = 0; q < 4; q++) {
+            const int c = col0 + (vb0 + i) * 16 + kq + 4 * q;
+            if (c >= g.ncol) continue;
+            const int oi = out_idx[c];
+            if (oi < 0) continue;
+            const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int r = row0 + (rb0 + j) * 16 + n;
+                if (r >= S) continue;
+                const double v = sc * acc[i][j][q];
+                double* y = &Y[(int64_t)oi * S + r];
+                *y = g.accumulate ? *y + v : v;
+            }
         }
-    }
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, const double* __restrict__ part,

@@ -130,8 +130,8 @@ __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
 }
 
 // Hadamard 8x8 (src/kernel.cpp:1012-1082, include/c2c.h:201-296) on the matrix cores.  Targets
-// t < ntar, source table rows slot_src_T[p * ntab + t] (-1: no source).  For one frequency f:
-//   Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
+// t < ntar, source table rows slot_src_T[p * ntab + t] (-1: no source; sources < ntab).  For one
+// frequency f:  Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
 // Each complex 8x8 G_p acts as the real 16x16 matrix M on interleaved (re, im) vectors:
 //   M[2it][2is] = M[2it+1][2is+1] = Re G[is][it],  M[2it][2is+1] = -Im G,  M[2it+1][2is] = Im G.
 // A wave multiplies M (A operand, rows = output reals) by 16 targets' source vectors (B operand,
@@ -139,56 +139,80 @@ __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
 // vector, so the K order is permuted (step kk, lane group kr -> real 4kr + kk) identically in A and
 // B.  The A rows are permuted too, so that output register i of lane (n, kr) is real 4kr + i of
 // target n whatever the accumulator layout of the type: A row d_row(g, i) holds output real 4g + i.
-// TT tiles of 16 targets per wave, 4 waves per workgroup, one frequency per workgroup; blocks are
-// remapped so that consecutive frequency slices run on one XCD.
+// The workgroup first writes each lane's four A values of all 26 patterns to LDS (one 4-vector read
+// per pattern afterwards).  The 26 patterns are software-pipelined: source indices are read two
+// patterns ahead and source vectors one pattern ahead, from row 0 when there is no source (the
+// vector is zeroed before use), so no load waits on a branch.  TT tiles of 16 targets per wave,
+// 4 waves per workgroup, one frequency per workgroup; blocks are remapped so that consecutive
+// frequency slices run on one XCD.
 template <typename T, int TT>
 __global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int ntab, int nchunk, int total,
                                                         const int32_t* __restrict__ slot_src_T,
                                                         const T* __restrict__ gk8, const T* __restrict__ Qk,
                                                         T* __restrict__ Pk) {
-    __shared__ T gs[26 * 128];
+    __shared__ vec4<T> as[26 * 64];
     const int per = (total + 7) / 8;
     const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
     if (Lb >= total) return;                                    // uniform over the workgroup
     const int f = Lb / nchunk, chunk = Lb % nchunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < 26 * 128; e += 256) {
-        const int p = e >> 7, m = e & 127;
-        gs[e] = gk8[((size_t)p * F + f) * 128 + m];
+    for (int e = tid; e < 26 * 64; e += 256) {                 // A operands of (pattern, lane)
+        const int p = e >> 6, l = e & 63, nl = l & 15, kl = l >> 4;
+        int o = nl;                                             // output real o = 2 it + (re 0 / im 1) of A row nl
+        for (int g = 0; g < 4; g++)
+            for (int i = 0; i < 4; i++)
+                if (d_row<T>(g, i) == nl) o = 4 * g + i;
+        const int it = o >> 1, im = o & 1;
+        const T* gp = gk8 + ((size_t)p * F + f) * 128;
+        const T g0r = gp[((2 * kl) * 8 + it) * 2], g0i = gp[((2 * kl) * 8 + it) * 2 + 1];
+        const T g1r = gp[((2 * kl + 1) * 8 + it) * 2], g1i = gp[((2 * kl + 1) * 8 + it) * 2 + 1];
+        as[e] = im ? vec4<T>{g0i, g0r, g1i, g1r} : vec4<T>{g0r, -g0i, g1r, -g1i};
     }
     __syncthreads();
     const int n = lane & 15, kr = lane >> 4;
-    // A row n carries output real o = 4g + i where d_row(g, i) = n; o = 2 it + (re 0 / im 1)
-    int o = n;
-    for (int g = 0; g < 4; g++)
-        for (int i = 0; i < 4; i++)
-            if (d_row<T>(g, i) == n) o = 4 * g + i;
-    const int itA = o >> 1, imA = o & 1;
     const int tbase = (chunk * 4 + wave) * 16 * TT;
-    vec4<T> acc[TT];
+    // 32-bit element offsets: the launcher checks ntab * F * 4 < 2^32
+    const vec4<T>* q4 = reinterpret_cast<const vec4<T>*>(Qk) + (size_t)f * 4 + kr;   // + src * F * 4
+    const uint32_t rowstride = (uint32_t)F * 4u;
+    int tcl[TT];
+#pragma unroll
+    for (int j = 0; j < TT; j++) tcl[j] = min(tbase + 16 * j + n, ntar - 1);   // lanes past ntar store nothing
+    auto load_idx = [&](int p, int* sidx) {
+#pragma unroll
+        for (int j = 0; j < TT; j++) sidx[j] = slot_src_T[(size_t)p * ntab + tcl[j]];
+    };
+    auto load_vec = [&](const int* sidx, vec4<T>* b) {
+#pragma unroll
+        for (int j = 0; j < TT; j++) b[j] = q4[(uint32_t)max(sidx[j], 0) * rowstride];
+    };
+    vec4<T> acc[TT], bv[TT], bn[TT];
+    int s0[TT], s1[TT], s2[TT];
 #pragma unroll
     for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
+    load_idx(0, s0);
+    load_vec(s0, bv);
+    load_idx(1, s1);
+#pragma unroll   // the buffer rotation below becomes register renaming
     for (int p = 0; p < 26; p++) {
-        const T* gp = gs + p * 128;
-        const T g0r = gp[((2 * kr) * 8 + itA) * 2], g0i = gp[((2 * kr) * 8 + itA) * 2 + 1];
-        const T g1r = gp[((2 * kr + 1) * 8 + itA) * 2], g1i = gp[((2 * kr + 1) * 8 + itA) * 2 + 1];
-        const T a0 = imA ? g0i : g0r, a1 = imA ? g0r : -g0i, a2 = imA ? g1i : g1r, a3 = imA ? g1r : -g1i;
-        vec4<T> bv[TT];
+        // keep the stages in place: the scheduler would otherwise hoist the next pattern's address
+        // arithmetic, and with it a wait on the loads just issued, into this one
+        __builtin_amdgcn_sched_barrier(0);
+        if (p + 1 < 26) load_vec(s1, bn);
+        if (p + 2 < 26) load_idx(p + 2, s2);
+        __builtin_amdgcn_sched_barrier(0);   // issue those loads before this pattern's matrix ops
+        const vec4<T> a = as[p * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < TT; j++)
+            if (s0[j] < 0) bv[j] = vec4<T>{0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a[k], bv[j][k], acc[j]);
 #pragma unroll
         for (int j = 0; j < TT; j++) {
-            const int t = tbase + 16 * j + n;
-            const int s = t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
-            if (s >= 0)
-                bv[j] = *reinterpret_cast<const vec4<T>*>(Qk + ((size_t)s * F + f) * 16 + 4 * kr);
-            else
-                bv[j] = vec4<T>{0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int j = 0; j < TT; j++) {
-            acc[j] = mfma16x16x4(a0, bv[j][0], acc[j]);
-            acc[j] = mfma16x16x4(a1, bv[j][1], acc[j]);
-            acc[j] = mfma16x16x4(a2, bv[j][2], acc[j]);
-            acc[j] = mfma16x16x4(a3, bv[j][3], acc[j]);
+            bv[j] = bn[j];
+            s0[j] = s1[j];
+            s1[j] = s2[j];
         }
     }
 #pragma unroll
@@ -391,6 +415,7 @@ template <typename T>
 hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
                        const void* Qk, void* Pk, hipStream_t st) {
     constexpr int TT = 4;
+    if ((uint64_t)ntab * (uint64_t)F * 4u >= (uint64_t(1) << 32)) return hipErrorInvalidValue;   // 32-bit offsets
     const int nchunk = (ntar + 64 * TT - 1) / (64 * TT);
     const int total = nchunk * F;
     const int per = (total + 7) / 8;
