@@ -14,8 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fmm3dpp.h"
@@ -265,6 +267,201 @@ inline void store(Bodies3& out, const std::vector<real>& phi, const std::vector<
     }
 }
 }  // namespace detail
+
+// ---- tree and interaction lists: the reference's Tree and Traverser (include/tree.h:9-75,
+//      include/traverser.h:9-56), built on the host by a structure-only plan (no GPU needed), with
+//      the same cell numbering (BFS, image cells appended) and list contents as the evaluation ----
+struct Range {   // the reference's OffsetAndNumber (include/type.h:550-565)
+    int offset = 0;
+    int number = 0;
+    Range() = default;
+    Range(int o, int n) : offset(o), number(n) {}
+};
+
+struct Cell3 {   // include/tree.h:9-30 without the per-cell q_equiv / p_check (kept in HBM here)
+    int idx = 0;
+    int octant = 0;   // 0..7 below a parent, 13 for the single child of an image cell
+    int depth = 0;
+    real r = 0;
+    vec3r x;
+    Range crange;
+    Range brange;
+};
+using Cells3 = std::vector<Cell3>;
+
+namespace detail {
+// A structure-only plan (FMM3DPP_DEVICE_NONE) for one box and leaf size.
+class StructurePlan {
+public:
+    StructurePlan(int ncrit, int images, real cycle, const vec3r& center) {
+        fmm3dpp_config cfg{};
+        cfg.P = 4;   // the structure does not depend on P
+        cfg.ncrit = ncrit;
+        cfg.images = images;
+        cfg.cycle = cycle;
+        for (int d = 0; d < 3; d++) cfg.center[d] = center[d];
+        cfg.precision = FMM3DPP_PREC_FP64;
+        cfg.dipole_mode = FMM3DPP_DIPOLE_OFF;
+        if (fmm3dpp_plan_create(&cfg, FMM3DPP_DEVICE_NONE, &plan_) != FMM3DPP_OK)
+            throw std::runtime_error("tree: invalid box or leaf size");
+    }
+    ~StructurePlan() { fmm3dpp_plan_destroy(plan_); }
+    StructurePlan(const StructurePlan&) = delete;
+    StructurePlan& operator=(const StructurePlan&) = delete;
+
+    void set(const Bodies3& bs) {
+        std::vector<real> x, y, z, q;
+        soa(bs, x, y, z, q);
+        check(fmm3dpp_set_particles(plan_, (int64_t)bs.size(), x.data(), y.data(), z.data()));
+        check(fmm3dpp_get_counts(plan_, &counts_));
+    }
+    std::vector<int32_t> order() {
+        std::vector<int32_t> perm(counts_.n_bodies);
+        check(fmm3dpp_get_body_order(plan_, perm.data()));
+        return perm;
+    }
+    Cells3 cells(bool with_images) {
+        std::vector<int32_t> ints(7 * (size_t)counts_.n_cells);
+        std::vector<double> dbl(4 * (size_t)counts_.n_cells);
+        check(fmm3dpp_get_cells(plan_, ints.data(), dbl.data()));
+        const int64_t nc = with_images ? counts_.n_cells : counts_.n_tree_cells;
+        Cells3 out((size_t)nc);
+        for (int64_t c = 0; c < nc; c++) {
+            const int32_t* I = &ints[7 * c];
+            const double* D = &dbl[4 * c];
+            Cell3& cell = out[c];
+            cell.idx = I[0];
+            cell.octant = I[1];
+            cell.depth = I[2];
+            cell.crange = Range(I[3], I[4]);
+            cell.brange = Range(I[5], I[6]);
+            cell.r = D[0];
+            cell.x = vec3r(D[1], D[2], D[3]);
+        }
+        return out;
+    }
+    const fmm3dpp_counts& counts() const { return counts_; }
+    fmm3dpp_plan* plan() { return plan_; }
+    void check(int rc) {
+        if (rc != FMM3DPP_OK) throw std::runtime_error(std::string("fmm3dpp: ") + fmm3dpp_last_error(plan_));
+    }
+
+private:
+    fmm3dpp_plan* plan_ = nullptr;
+    fmm3dpp_counts counts_{};
+};
+}  // namespace detail
+
+class Tree {
+public:
+    enum class TreeType { uniform, nonuniform };
+
+    // Adaptive octree of box (x, r) with at most m bodies per leaf (src/tree.cpp:102-188); reorders
+    // `bodies` into tree order as the reference does.  Only the nonuniform tree is on the
+    // evaluation path (src/fmm.cpp:19).
+    void build(Bodies3& bodies, vec3r x, real r, int m, TreeType type) {
+        if (type != TreeType::nonuniform)
+            throw std::runtime_error("tree: only the adaptive (nonuniform) octree of the evaluation path is built");
+        detail::StructurePlan sp(m, 0, 2 * r, x);
+        sp.set(bodies);
+        const std::vector<int32_t> perm = sp.order();
+        Bodies3 sorted(bodies.size());
+        for (size_t i = 0; i < sorted.size(); i++) sorted[i] = bodies[perm[i]];
+        bodies.swap(sorted);
+        cells_ = sp.cells(false);
+        bodies_ = bodies;
+        x_ = x;
+        r_ = r;
+        ncrit_ = m;
+    }
+    Cells3 get_cells() { return cells_; }
+
+    // Centre of child `octant` of a cell (src/tree.cpp:195-219): octant bits x = 4, y = 2, z = 1;
+    // periodic children 0..26 of an image cell sit on a 3x3x3 grid of spacing 2 r / 3.
+    static vec3r get_child_cell_x(vec3r x_par, real r_par, int octant, int is_periodic) {
+        vec3r x;
+        if (!is_periodic) {
+            if (octant < 0 || octant > 7) throw std::runtime_error("octant out of range");
+            for (int d = 0; d < 3; d++) x[d] = x_par[d] + ((octant >> (2 - d)) & 1 ? r_par / 2 : -r_par / 2);
+        } else {
+            if (octant < 0 || octant > 26) throw std::runtime_error("periodic octant out of range");
+            const int k[3] = {octant % 3 - 1, (octant % 9) / 3 - 1, octant / 9 - 1};
+            for (int d = 0; d < 3; d++) x[d] = x_par[d] + k[d] * (r_par * 2 / 3.0);
+        }
+        return x;
+    }
+
+    const Bodies3& bodies() const { return bodies_; }
+    vec3r center() const { return x_; }
+    real radius() const { return r_; }
+    int ncrit() const { return ncrit_; }
+
+private:
+    Cells3 cells_;
+    Bodies3 bodies_;
+    vec3r x_;
+    real r_ = 0;
+    int ncrit_ = 0;
+};
+
+enum class OperatorType { P2P, M2L, M2P, P2L };
+using PeriodicInteractionPair = std::pair<int, std::pair<int, vec3r>>;
+using PeriodicInteractionPairs = std::vector<PeriodicInteractionPair>;
+using PeriodicInteractionMap = std::map<int, std::vector<std::pair<int, vec3r>>>;
+
+struct PeriodicParentSource {   // include/traverser.h:17-23
+    PeriodicParentSource(int idx_, vec3r offset_, int is_single_parent_)
+        : idx(idx_), offset(offset_), is_single_parent(is_single_parent_) {}
+    int idx;
+    vec3r offset;
+    int is_single_parent;
+};
+using PeriodicM2LMap = std::map<int, std::vector<PeriodicParentSource>>;
+
+// Interaction lists of a tree (src/traverser.cpp:19-68): the P2P list of every leaf with periodic
+// offsets and the M2L parent map (same-depth adjacent non-leaf parents, image cells' 26 self
+// entries).  The image cells c(-1..-images) are appended to the cells.  cycle must equal the tree's
+// box size 2 r when images >= 1 (the reference's driver always builds it so, src/argument.cpp:61-62).
+class Traverser {
+public:
+    void traverse(Tree& tree, real cycle = 2 * M_PI, int images = 0) {
+        if (images >= 1 && std::fabs(cycle - 2 * tree.radius()) > 1e-12 * cycle)
+            throw std::runtime_error("traverser: cycle must equal the tree's box size 2 r");
+        detail::StructurePlan sp(tree.ncrit(), images, 2 * tree.radius(), tree.center());
+        sp.set(tree.bodies());
+        cells_ = sp.cells(true);
+        const fmm3dpp_counts& c = sp.counts();
+        std::vector<int32_t> tar(c.n_p2p), src(c.n_p2p);
+        std::vector<double> off(3 * (size_t)c.n_p2p);
+        sp.check(fmm3dpp_get_p2p(sp.plan(), tar.data(), src.data(), off.data()));
+        p2p_.clear();
+        for (int64_t i = 0; i < c.n_p2p; i++)
+            p2p_.push_back({tar[i], {src[i], vec3r(off[3 * i], off[3 * i + 1], off[3 * i + 2])}});
+        std::vector<int32_t> mt(c.n_m2l), ms(c.n_m2l), mi(c.n_m2l);
+        std::vector<double> mo(3 * (size_t)c.n_m2l);
+        sp.check(fmm3dpp_get_m2l(sp.plan(), mt.data(), ms.data(), mo.data(), mi.data()));
+        m2l_.clear();
+        for (int64_t i = 0; i < c.n_m2l; i++)
+            m2l_[mt[i]].push_back(PeriodicParentSource(ms[i], vec3r(mo[3 * i], mo[3 * i + 1], mo[3 * i + 2]), mi[i]));
+    }
+    PeriodicInteractionPairs get_pairs(OperatorType type) {
+        if (type != OperatorType::P2P) throw std::runtime_error("traverser: only P2P pairs are on the evaluation path");
+        return p2p_;
+    }
+    PeriodicInteractionMap get_map(OperatorType type) {
+        if (type != OperatorType::P2P) throw std::runtime_error("traverser: only the P2P map is on the evaluation path");
+        PeriodicInteractionMap m;
+        for (const auto& pr : p2p_) m[pr.first].push_back(pr.second);
+        return m;
+    }
+    PeriodicM2LMap get_M2L_parent_map() { return m2l_; }
+    Cells3 get_cells() { return cells_; }
+
+private:
+    Cells3 cells_;
+    PeriodicInteractionPairs p2p_;
+    PeriodicM2LMap m2l_;
+};
 
 // LaplaceKernel::direct (src/kernel.cpp:11-77) on the GPU: bs_tar.p/f += the direct lattice sum of
 // bs_src over (3^images)^3 copies; sources and targets are the same bodies, as the reference's
