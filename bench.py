@@ -35,6 +35,11 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    """Progress on stderr (the JSON result line is the only stdout output)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def workload(n, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     pts = rng.random((n, 3)) - 0.5
@@ -252,7 +257,9 @@ def main():
                           "breakdown_ms": r["stages"]}), flush=True)
         return
     r64 = run_path(pyfmm3dpp, torch, args, pyfmm3dpp.PREC_FP64, d_x, d_y, d_z, d_q)
+    log(f"float64 path {r64['ms_per_step']:.3f} ms/step")
     rmx = run_path(pyfmm3dpp, torch, args, pyfmm3dpp.PREC_MIXED, d_x, d_y, d_z, d_q)
+    log(f"mixed path {rmx['ms_per_step']:.3f} ms/step")
     main_run = rmx if args.path == "mixed" else r64
     value = args.n / (main_run["ms_per_step"] / 1e3)
     v64 = args.n / (r64["ms_per_step"] / 1e3)
@@ -269,6 +276,7 @@ def main():
                                               d_q.data_ptr(), len(tar), d_tar.data_ptr(), 1.0, 23, 12.0, 0.5,
                                               d_pe.data_ptr(), d_ge.data_ptr())
     ewald_s = time.perf_counter() - t0
+    log(f"Ewald reference on {len(tar)} targets {ewald_s:.2f} s")
     if rc != 0:
         raise RuntimeError(f"fmm3dpp_ewald_device failed ({rc})")
     idx = torch.from_numpy(tar).to(dev)
@@ -327,11 +335,14 @@ def main():
         "precompute_s": round(main_run["precompute_s"], 3),
         "roofline": roofline,
     }
+    log("accuracy legs done")
     if not args.no_batch:
         out["batch"] = run_batch(pyfmm3dpp, torch, args)
+        log(f"batch leg: {out['batch']['speedup']}x over one plan per system")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(x, y, z, q, args.P, args.ncrit, args.images, threads, 30)
+        log(f"CPU baseline {out['cpu_baseline']['value']:.3g} particles/s on {threads} threads")
     print(json.dumps(out), flush=True)
 
 
