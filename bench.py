@@ -155,6 +155,17 @@ def run_batch(pyfmm3dpp, torch, args):
             "rel_l2_batch_vs_sequential": float("%.3e" % diff)}
 
 
+def profiled_traffic(path):
+    """HBM bytes per launch of the dominant kernel from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE, each divided by the same counter's reading on a copy kernel of known size), or None."""
+    f = os.path.join(ROOT, "profiles", f"hadamard8_{path}_N1M_P8_hbm_traffic.md")
+    if not os.path.exists(f):
+        return None
+    import re
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)", open(f).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 STAGES = ["P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize", "evaluate"]
 
 
@@ -305,7 +316,8 @@ def main():
     peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
     roofline = {"kernel": "k_hadamard8_mfma<%s> (M2L 8x8 Hadamard on the matrix cores)" % ("float" if mixed else "double"),
                 "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None, "flops_per_launch": flops, "pairs": pairs,
+                "frac": round(achieved / peak, 4), "traffic": profiled_traffic(args.path),
+                "flops_per_launch": flops, "pairs": pairs,
                 "launch_ms": had_ms}
 
     out = {
