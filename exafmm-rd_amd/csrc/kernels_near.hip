@@ -42,38 +42,69 @@ __device__ __forceinline__ T fold_groups(T v, int wbits) {
 
 // P2P, float64 (src/kernel.cpp:258-311 p2p_1toN_256 over the leaf's P2P list, fmm.cpp:259-294):
 // phi_t += q_s / r, grad_t -= q_s (x_t - x_s) / r^3 with x_s shifted by the list entry's periodic
-// offset.  One wave per target leaf.
+// offset.  One wave per target leaf.  The list is walked in blocks of 64 entries whose source
+// ranges go to LDS first; each source leaf is then read in pieces of up to 64 bodies, one body per
+// lane into LDS, with the next piece's loads issued before the current piece is summed.
 __global__ __launch_bounds__(64) void k_p2p_f64(const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
                                                 const int64_t* __restrict__ list_off,
                                                 const int32_t* __restrict__ list_ent, double cycle,
                                                 double* __restrict__ p, double* __restrict__ gx,
                                                 double* __restrict__ gy, double* __restrict__ gz) {
+    __shared__ int m_sb[64], m_ns[64], m_oi[64];
+    __shared__ double4 sbuf[64];
     const int li = blockIdx.x, lane = threadIdx.x;
     const int c = leaf_cell[li];
     const int boff = cells.boff[c], bnum = cells.bnum[c];
     const int64_t e0 = list_off[li], e1 = list_off[li + 1];
     const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
     const int g = lane >> wb, tl = lane & (W - 1);
+    auto piece = [&](int k, int k0) {   // this lane's body of piece (entry k, bodies k0..k0+63)
+        const int j = k0 + lane;
+        if (j >= m_ns[k]) return make_double4(0, 0, 0, 0);
+        const int idx = m_sb[k] + j;
+        return make_double4(b.x[idx], b.y[idx], b.z[idx], b.q[idx]);
+    };
     for (int t0 = 0; t0 < bnum; t0 += W) {
         const int i = t0 + tl;
         const bool active = i < bnum;
         const int bi = boff + (active ? i : 0);
         const double xt = b.x[bi], yt = b.y[bi], zt = b.z[bi];
         double ap = 0, ax = 0, ay = 0, az = 0;
-        for (int64_t e = e0; e < e1; e++) {
-            const int ent = list_ent[e], sc = ent >> 5, oi = ent & 31;   // source cell, periodic offset index
-            const double xo = xt - (oi % 3 - 1) * cycle, yo = yt - ((oi / 3) % 3 - 1) * cycle,
-                         zo = zt - (oi / 9 - 1) * cycle;
-            const int sb = cells.boff[sc], se = sb + cells.bnum[sc];
-            for (int j = sb + g; j < se; j += G) {
-                const double dx = xo - b.x[j], dy = yo - b.y[j], dz = zo - b.z[j];
-                const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
-                double qi = b.q[j] * ir;
-                ap += qi;
-                qi *= ir * ir;
-                ax += qi * dx;
-                ay += qi * dy;
-                az += qi * dz;
+        for (int64_t eb = e0; eb < e1; eb += 64) {
+            const int ne = (int)min((int64_t)64, e1 - eb);
+            __syncthreads();
+            if (lane < ne) {
+                const int ent = list_ent[eb + lane], sc = ent >> 5;   // source cell, periodic offset index
+                m_sb[lane] = cells.boff[sc];
+                m_ns[lane] = cells.bnum[sc];
+                m_oi[lane] = ent & 31;
+            }
+            __syncthreads();
+            int k = 0, k0 = 0;
+            double4 r = piece(0, 0);
+            while (k < ne) {
+                sbuf[lane] = r;
+                __syncthreads();
+                int nk = k, nk0 = k0 + 64;
+                if (nk0 >= m_ns[k]) { nk++; nk0 = 0; }
+                if (nk < ne) r = piece(nk, nk0);   // in flight while this piece is summed
+                const int oi = m_oi[k], cnt = min(64, m_ns[k] - k0);
+                const double xo = xt - (oi % 3 - 1) * cycle, yo = yt - ((oi / 3) % 3 - 1) * cycle,
+                             zo = zt - (oi / 9 - 1) * cycle;
+                for (int j = g; j < cnt; j += G) {
+                    const double4 sv = sbuf[j];
+                    const double dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
+                    const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+                    double qi = sv.w * ir;
+                    ap += qi;
+                    qi *= ir * ir;
+                    ax += qi * dx;
+                    ay += qi * dy;
+                    az += qi * dz;
+                }
+                __syncthreads();
+                k = nk;
+                k0 = nk0;
             }
         }
         ap = fold_groups(ap, wb);
@@ -102,15 +133,19 @@ __global__ __launch_bounds__(64) void k_pack_rel(const int32_t* __restrict__ lea
     }
 }
 
-// P2P, float32 with two-level summation: the pairs of one (target, source leaf) are summed in
-// float32, each source leaf's partial is added to float64 accumulators.  The source leaf's
+// P2P, float32 with two-level summation: the pairs of one (target, piece of a source leaf) are
+// summed in float32, each piece's partial is added to float64 accumulators.  The source leaf's
 // displacement (its centre + periodic offset - the target leaf centre) is formed in float64 once
-// per list entry, so no float32 coordinate carries an absolute position.
+// per list entry, so no float32 coordinate carries an absolute position.  List blocks and source
+// pieces are staged through LDS as in k_p2p_f64.
 __global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf_cell, CellsDev cells,
                                                 const float4* __restrict__ rel, const int64_t* __restrict__ list_off,
                                                 const int32_t* __restrict__ list_ent, double cycle,
                                                 double* __restrict__ p, double* __restrict__ gx,
                                                 double* __restrict__ gy, double* __restrict__ gz) {
+    __shared__ int m_sb[64], m_ns[64];
+    __shared__ float m_dx[64], m_dy[64], m_dz[64];
+    __shared__ float4 sbuf[64];
     const int li = blockIdx.x, lane = threadIdx.x;
     const int c = leaf_cell[li];
     const int boff = cells.boff[c], bnum = cells.bnum[c];
@@ -118,34 +153,58 @@ __global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf
     const int64_t e0 = list_off[li], e1 = list_off[li + 1];
     const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
     const int g = lane >> wb, tl = lane & (W - 1);
+    auto piece = [&](int k, int k0) {
+        const int j = k0 + lane;
+        return j < m_ns[k] ? rel[m_sb[k] + j] : make_float4(0, 0, 0, 0);
+    };
     for (int t0 = 0; t0 < bnum; t0 += W) {
         const int i = t0 + tl;
         const bool active = i < bnum;
         const int bi = boff + (active ? i : 0);
         const float4 tv = rel[bi];
         double ap = 0, ax = 0, ay = 0, az = 0;
-        for (int64_t e = e0; e < e1; e++) {
-            const int ent = list_ent[e], sc = ent >> 5, oi = ent & 31;
-            const float xo = tv.x - (float)(cells.x[sc] + (oi % 3 - 1) * cycle - cx);
-            const float yo = tv.y - (float)(cells.y[sc] + ((oi / 3) % 3 - 1) * cycle - cy);
-            const float zo = tv.z - (float)(cells.z[sc] + (oi / 9 - 1) * cycle - cz);
-            const int sb = cells.boff[sc], se = sb + cells.bnum[sc];
-            float fp = 0, fx = 0, fy = 0, fz = 0;
-            for (int j = sb + g; j < se; j += G) {
-                const float4 sv = rel[j];
-                const float dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
-                const float ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
-                float qi = sv.w * ir;
-                fp += qi;
-                qi *= ir * ir;
-                fx += qi * dx;
-                fy += qi * dy;
-                fz += qi * dz;
+        for (int64_t eb = e0; eb < e1; eb += 64) {
+            const int ne = (int)min((int64_t)64, e1 - eb);
+            __syncthreads();
+            if (lane < ne) {
+                const int ent = list_ent[eb + lane], sc = ent >> 5, oi = ent & 31;
+                m_sb[lane] = cells.boff[sc];
+                m_ns[lane] = cells.bnum[sc];
+                m_dx[lane] = (float)(cells.x[sc] + (oi % 3 - 1) * cycle - cx);
+                m_dy[lane] = (float)(cells.y[sc] + ((oi / 3) % 3 - 1) * cycle - cy);
+                m_dz[lane] = (float)(cells.z[sc] + (oi / 9 - 1) * cycle - cz);
             }
-            ap += fp;
-            ax += fx;
-            ay += fy;
-            az += fz;
+            __syncthreads();
+            int k = 0, k0 = 0;
+            float4 r = piece(0, 0);
+            while (k < ne) {
+                sbuf[lane] = r;
+                __syncthreads();
+                int nk = k, nk0 = k0 + 64;
+                if (nk0 >= m_ns[k]) { nk++; nk0 = 0; }
+                if (nk < ne) r = piece(nk, nk0);   // in flight while this piece is summed
+                const int cnt = min(64, m_ns[k] - k0);
+                const float xo = tv.x - m_dx[k], yo = tv.y - m_dy[k], zo = tv.z - m_dz[k];
+                float fp = 0, fx = 0, fy = 0, fz = 0;
+                for (int j = g; j < cnt; j += G) {
+                    const float4 sv = sbuf[j];
+                    const float dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
+                    const float ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+                    float qi = sv.w * ir;
+                    fp += qi;
+                    qi *= ir * ir;
+                    fx += qi * dx;
+                    fy += qi * dy;
+                    fz += qi * dz;
+                }
+                ap += fp;
+                ax += fx;
+                ay += fy;
+                az += fz;
+                __syncthreads();
+                k = nk;
+                k0 = nk0;
+            }
         }
         ap = fold_groups(ap, wb);
         ax = fold_groups(ax, wb);
