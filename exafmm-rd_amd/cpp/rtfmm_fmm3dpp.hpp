@@ -484,6 +484,41 @@ public:
         detail::store(bs_tar, phi, grad);
     }
 
+    // p2p (src/kernel.cpp:79-100): the bodies of cell_tar in bs_tar get p += sum q/r and
+    // f += sum -q dx/r^3 over the bodies of cell_src in bs_src shifted by offset (dx = x_t - x_s -
+    // offset, r = 0 skipped), on the GPU.  use_simd only selects the reference's CPU code path.
+    void p2p(const Bodies3& bs_src, Bodies3& bs_tar, const Cell3& cell_src, const Cell3& cell_tar, vec3r offset,
+             int use_simd = 1) {
+        (void)use_simd;
+        const int ns = cell_src.brange.number, nt = cell_tar.brange.number;
+        if (ns <= 0 || nt <= 0) return;
+        if (cell_src.brange.offset < 0 || cell_src.brange.offset + ns > (int)bs_src.size() ||
+            cell_tar.brange.offset < 0 || cell_tar.brange.offset + nt > (int)bs_tar.size())
+            throw std::runtime_error("p2p: body range out of range");
+        // one body set: the shifted sources, then the targets with zero charge
+        const int n = ns + nt;
+        std::vector<real> x(n), y(n), z(n), q(n, 0.0);
+        for (int i = 0; i < ns; i++) {
+            const Body3& b = bs_src[cell_src.brange.offset + i];
+            x[i] = b.x[0] + offset[0]; y[i] = b.x[1] + offset[1]; z[i] = b.x[2] + offset[2]; q[i] = b.q;
+        }
+        std::vector<int32_t> tar(nt);
+        for (int j = 0; j < nt; j++) {
+            const Body3& b = bs_tar[cell_tar.brange.offset + j];
+            x[ns + j] = b.x[0]; y[ns + j] = b.x[1]; z[ns + j] = b.x[2];
+            tar[j] = ns + j;
+        }
+        std::vector<real> phi(nt), grad(3 * (size_t)nt);
+        int rc = fmm3dpp_direct(device_, n, x.data(), y.data(), z.data(), q.data(), nt, tar.data(), 1.0, 0,
+                                phi.data(), grad.data());
+        if (rc != FMM3DPP_OK) throw std::runtime_error("fmm3dpp_direct failed: " + std::to_string(rc));
+        for (int j = 0; j < nt; j++) {
+            Body3& b = bs_tar[cell_tar.brange.offset + j];
+            b.p += phi[j];
+            for (int d = 0; d < 3; d++) b.f[d] += grad[3 * (size_t)j + d];
+        }
+    }
+
 private:
     int device_;
 };
@@ -535,7 +570,7 @@ struct BodyCompareResult {
 };
 
 inline BodyCompareResult compare(const Bodies3& bs1, const Bodies3& bs2, std::string name1, std::string name2,
-                                 int num_compare) {
+                                 int num_compare = -1) {
     if (bs1.size() != bs2.size()) throw std::runtime_error("inconsistent size in comparison");
     BodyCompareResult res;
     res.name1 = name1;

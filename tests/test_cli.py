@@ -55,3 +55,28 @@ def test_reference_log_blocks_reproduced():
     assert abs(e2 - row["energy_ref"]) <= 1e-9 * abs(row["energy_ref"])
     l2f = float(re.search(r"L2  \(f\)  : (\S+)", block).group(1))
     assert abs(l2f - row["l2f"]) <= 1e-4 * row["l2f"]
+
+
+@pytest.mark.gpu
+def test_fmm_driver_nonperiodic_log():
+    """test_fmm (3dpp/test/test_fmm.cpp) with the logged free-space setting (setting_t: no dipole term,
+    no net-charge removal, 1/(4 pi)), P=6, N=24000, ncrit=128: FMM energy and direct energy as logged
+    (slurm-152673.out), L2(p) and L2(f) to their 6 printed digits."""
+    exe = os.path.join(os.path.dirname(HERE), "exafmm-rd_amd", "cpp", "test_fmm")
+    if not os.path.exists(exe):
+        pytest.skip("exafmm-rd_amd/cpp/test_fmm not built")
+    with open(os.path.join(HERE, "golden", "log_energies.json")) as f:
+        row = [r for r in json.load(f)["nonperiodic"] if r["P"] == 6][0]
+    r = subprocess.run([exe, "-P", "6", "-n", "24000", "-m", "128", "-i", "0", "--setting_t", "1"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout, flush=True)
+    assert r.stdout.startswith("rtfmm_3dp_test_fmm\n")
+    block = r.stdout[r.stdout.index("FMM vs Direct"):]
+    e1 = float(re.search(r"p-energy1 : (\S+)", block).group(1))
+    e2 = float(re.search(r"p-energy2 : (\S+)", block).group(1))
+    assert abs(e1 - row["energy_fmm"]) <= 2e-12 * abs(row["energy_fmm"])
+    assert abs(e2 - row["energy_ref"]) <= 1e-12 * abs(row["energy_ref"])
+    l2p = float(re.search(r"L2  \(p\)  : (\S+)", block).group(1))
+    l2f = float(re.search(r"L2  \(f\)  : (\S+)", block).group(1))
+    assert abs(l2p - row["l2p"]) <= 1e-4 * row["l2p"] and abs(l2f - row["l2f"]) <= 1e-4 * row["l2f"]
