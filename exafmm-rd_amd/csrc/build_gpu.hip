@@ -96,28 +96,41 @@ __global__ void k_keys(int64_t n, const double* __restrict__ x, const double* __
 }
 
 // Octant sub-ranges of every splitting cell of level l (cells [ls, le)).
+// Split of the cells of level l (tree.cpp:147-186): 8 lanes per cell, lane o finds the end of
+// octant o in the cell's key-sorted bodies by its own binary search on digit l, so the search depth
+// is log2(bodies) rather than 8 x that.  cstart[9 * lc + o] = first body of octant o, [8] = end.
 __global__ void k_split(int ls, int le, int l, int ncrit, const uint64_t* __restrict__ K1,
                         const int32_t* __restrict__ boff, const int32_t* __restrict__ bnum, int32_t* __restrict__ nchild,
                         int32_t* __restrict__ cstart, int* __restrict__ flag) {
-    int c = ls + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= le) return;
-    int lc = c - ls, b = boff[c], m = bnum[c];
-    if (m <= ncrit) { nchild[lc] = 0; return; }
-    if (l >= kD) { atomicOr(flag, 2); nchild[lc] = 0; return; }
-    const int shift = 3 * (kD - l - 1);
-    int pos = b, cnt = 0;
-    for (int o = 0; o < 8; o++) {
-        int lo = pos, hi = b + m;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int o = (int)(gt & 7);
+    const int64_t c64 = ls + (gt >> 3);
+    const bool valid = c64 < le;
+    const int c = valid ? (int)c64 : ls;   // lanes past the level keep the shuffles below well defined
+    const int lc = c - ls, b = boff[c], m = bnum[c];
+    const bool split = valid && m > ncrit && l < kD;
+    if (valid && m > ncrit && l >= kD && o == 0) atomicOr(flag, 2);
+    int ub = b + m;
+    if (split) {
+        const int shift = 3 * (kD - l - 1);
+        int lo = b, hi = b + m;   // first body whose digit exceeds o
         while (lo < hi) {
-            int mid = (lo + hi) >> 1;
+            const int mid = (lo + hi) >> 1;
             if ((int)((K1[mid] >> shift) & 7) <= o) lo = mid + 1; else hi = mid;
         }
-        cstart[lc * 9 + o] = pos;
-        if (lo > pos) cnt++;
-        pos = lo;
+        ub = lo;
     }
-    cstart[lc * 9 + 8] = b + m;
-    nchild[lc] = cnt;
+    // start of octant o = end of octant o - 1 (the cell's first body for o = 0)
+    int prev = __shfl_up(ub, 1, 8);
+    if (o == 0) prev = b;
+    int nonempty = split && ub > prev ? 1 : 0;
+    for (int d = 1; d < 8; d <<= 1) nonempty += __shfl_xor(nonempty, d, 8);
+    if (!valid) return;
+    if (split) {
+        cstart[lc * 9 + o] = prev;
+        if (o == 7) cstart[lc * 9 + 8] = b + m;
+    }
+    if (o == 0) nchild[lc] = split ? nonempty : 0;
 }
 
 __global__ void k_emit_children(int ls, int le, const int32_t* __restrict__ nchild, const int32_t* __restrict__ incl,
@@ -416,6 +429,44 @@ __global__ void k_csr_offsets(int nleaf, int64_t ne, const int32_t* __restrict__
     off[r] = lo;
 }
 
+// Root cell (tree.cpp:107-117): centre, half size r0, all bodies, octant 13.
+__global__ void k_root_init(double c0x, double c0y, double c0z, double r0, int32_t n, uint64_t* __restrict__ prefix,
+                            double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
+                            double* __restrict__ cr, int32_t* __restrict__ depth, int32_t* __restrict__ octant,
+                            int32_t* __restrict__ boff, int32_t* __restrict__ bnum, int32_t* __restrict__ coff,
+                            int32_t* __restrict__ cnum) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    prefix[0] = 0;
+    cx[0] = c0x; cy[0] = c0y; cz[0] = c0z; cr[0] = r0;
+    depth[0] = 0; octant[0] = 13; boff[0] = 0; bnum[0] = n; coff[0] = 0; cnum[0] = 0;
+}
+
+// dst[1] = a[idx] (the total of an inclusive scan next to the error flag dst[0], read back together)
+__global__ void k_copy_last(const int32_t* __restrict__ a, int64_t idx, int32_t* __restrict__ dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) dst[1] = a[idx];
+}
+
+// M2L tables of the image targets t = nl + m - 1, m = 1..nimg (the parents c(-1..-(images-1))):
+// one child (slot 0: the root for m = 1, else c(-(m-1))), scale sc[m-1], no 8x8 sources and the
+// 26 self entries of the 1x27 lattice kernel (traverser.cpp:43-64).
+__global__ void k_image_targets(int nl, int nimg, int ntree, int ntar, const double* __restrict__ sc,
+                                int32_t* __restrict__ nl_cells, int32_t* __restrict__ child_slots,
+                                double* __restrict__ tar_scale, int32_t* __restrict__ slot_src_T,
+                                int32_t* __restrict__ img_src_T, int32_t* __restrict__ img_tar) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nimg) return;
+    const int m = i + 1, t = nl + i;
+    img_tar[i] = t;
+    nl_cells[t] = ntree + m - 1;
+    for (int o = 0; o < 8; o++) child_slots[(size_t)t * 8 + o] = -1;
+    child_slots[(size_t)t * 8] = m == 1 ? 0 : ntree + m - 2;
+    tar_scale[t] = sc[i];
+    for (int p = 0; p < 26; p++) {
+        slot_src_T[(size_t)p * ntar + t] = -1;
+        img_src_T[(size_t)p * ntar + t] = t;
+    }
+}
+
 __global__ void k_image_cells(int ntree, int images, int64_t n, double* __restrict__ cx, double* __restrict__ cy,
                               double* __restrict__ cz, double* __restrict__ cr, int32_t* __restrict__ depth,
                               int32_t* __restrict__ octant, int32_t* __restrict__ coff, int32_t* __restrict__ cnum,
@@ -503,34 +554,26 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
         return hipSuccess;
     };
     CK(grow_cells(std::max<size_t>(1024, (size_t)(n / std::max(1, cfg.ncrit)) * 3)));
-    {   // root (tree.cpp:107-117)
-        uint64_t p0 = 0; double c0[3] = {cfg.center[0], cfg.center[1], cfg.center[2]};
-        int32_t zero = 0, thirteen = 13, nn = (int32_t)n;
-        CK(hipMemcpyAsync(prefix.p, &p0, 8, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(cx.p, &c0[0], 8, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(cy.p, &c0[1], 8, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(cz.p, &c0[2], 8, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(cr.p, &r0, 8, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(depth.p, &zero, 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(octant.p, &thirteen, 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(boff.p, &zero, 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(bnum.p, &nn, 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(coff.p, &zero, 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(cnum.p, &zero, 4, hipMemcpyHostToDevice, st));
-        CK(hipStreamSynchronize(st));
-    }
+    k_root_init<<<1, 1, 0, st>>>(cfg.center[0], cfg.center[1], cfg.center[2], r0, (int32_t)n, prefix.as<uint64_t>(),
+                                 cx.as<double>(), cy.as<double>(), cz.as<double>(), cr.as<double>(),
+                                 depth.as<int32_t>(), octant.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
+                                 coff.as<int32_t>(), cnum.as<int32_t>());
+    CK(hipGetLastError());
     level_start.assign(1, 0);
     level_start.push_back(1);
     for (int l = 0;; l++) {
         int ls = level_start[l], le = level_start[l + 1], ncl = le - ls;
         CK(nchild.ensure((size_t)ncl * 4)); CK(scan32.ensure((size_t)ncl * 4)); CK(cstart.ensure((size_t)ncl * 9 * 4));
-        k_split<<<blocks(ncl), 256, 0, st>>>(ls, le, l, cfg.ncrit, key_s.as<uint64_t>(), boff.as<int32_t>(),
+        k_split<<<blocks((int64_t)ncl * 8), 256, 0, st>>>(ls, le, l, cfg.ncrit, key_s.as<uint64_t>(), boff.as<int32_t>(),
                                               bnum.as<int32_t>(), nchild.as<int32_t>(), cstart.as<int32_t>(), flag);
         CK(hipGetLastError());
         CK(scan_incl(temp, nchild.as<int32_t>(), scan32.as<int32_t>(), ncl, st));
-        int32_t total = 0;
-        CK(read_back(scan32.as<int32_t>() + (ncl - 1), &total, 1, st));
-        CK(read_back(flag, &hflag, 1, st));
+        k_copy_last<<<1, 1, 0, st>>>(scan32.as<int32_t>(), ncl - 1, flag);
+        CK(hipGetLastError());
+        int32_t fl[2] = {0, 0};   // error flag, children total
+        CK(read_back(flag, fl, 2, st));
+        hflag = fl[0];
+        const int32_t total = fl[1];
         if (hflag & 2) { err = "more than ncrit bodies share a cell at depth 21 (coincident bodies?)"; return FMM3DPP_ERR_TREE; }
         CK(grow_cells((size_t)le + total));
         k_emit_children<<<blocks(ncl), 256, 0, st>>>(ls, le, nchild.as<int32_t>(), scan32.as<int32_t>(),
@@ -646,26 +689,15 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
         CK(hipGetLastError());
     }
     if (nimg > 0) {   // image targets c(-1..-(images-1)): 26 self entries through the 1x27 lattice kernel
-        std::vector<int32_t> it(nimg), cells_(nimg), slots(8 * (size_t)nimg, -1);
         std::vector<double> sc(nimg);
-        for (int m = 1; m <= nimg; m++) {
-            int t = nl + m - 1;
-            it[m - 1] = t;
-            cells_[m - 1] = ntree + m - 1;
-            slots[(size_t)(m - 1) * 8] = m == 1 ? 0 : ntree + m - 2;
-            sc[m - 1] = std::pow(-m >= -1 ? 2.0 : 3.0, -m + 1);   // kernel.cpp:1372
-        }
-        CK(upload(img_tar, it, st));
-        CK(hipMemcpyAsync(nl_cells.as<int32_t>() + nl, cells_.data(), nimg * 4, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(child_slots.as<int32_t>() + (size_t)nl * 8, slots.data(), (size_t)nimg * 32, hipMemcpyHostToDevice, st));
-        CK(hipMemcpyAsync(tar_scale.as<double>() + nl, sc.data(), nimg * 8, hipMemcpyHostToDevice, st));
-        std::vector<int32_t> minus(nimg, -1), self(nimg);
-        for (int m = 0; m < nimg; m++) self[m] = nl + m;
-        for (int p = 0; p < 26; p++) {
-            CK(hipMemcpyAsync(slot_src_T.as<int32_t>() + (size_t)p * ntar + nl, minus.data(), nimg * 4, hipMemcpyHostToDevice, st));
-            CK(hipMemcpyAsync(img_src_T.as<int32_t>() + (size_t)p * ntar + nl, self.data(), nimg * 4, hipMemcpyHostToDevice, st));
-        }
-        CK(hipStreamSynchronize(st));
+        for (int m = 1; m <= nimg; m++) sc[m - 1] = std::pow(-m >= -1 ? 2.0 : 3.0, -m + 1);   // kernel.cpp:1372
+        CK(upload(img_scale, sc, st));
+        CK(img_tar.ensure((size_t)nimg * 4));
+        k_image_targets<<<blocks(nimg), 256, 0, st>>>(nl, nimg, ntree, ntar, img_scale.as<double>(),
+                                                       nl_cells.as<int32_t>(), child_slots.as<int32_t>(),
+                                                       tar_scale.as<double>(), slot_src_T.as<int32_t>(),
+                                                       img_src_T.as<int32_t>(), img_tar.as<int32_t>());
+        CK(hipGetLastError());
     } else {
         CK(img_tar.ensure(4));
     }

@@ -34,7 +34,7 @@ struct GpuStructure {
     // operator-apply column tables
     DBuf m2m_in, m2m_out, l2l_in, l2l_out, img_a, img_b;
     // M2L Hadamard tables (sources = targets = non-leaf cells, then image cells)
-    DBuf nl_idx, nl_cells, child_slots, tar_scale, slot_src_T, img_src_T, img_tar;
+    DBuf nl_idx, nl_cells, child_slots, tar_scale, slot_src_T, img_src_T, img_tar, img_scale;
     // P2P rows (target, source, offset index) and per-leaf CSR (source << 5 | offset index)
     DBuf rows_tc, rows_sc, rows_oi, p2p_off, p2p_ent;
     size_t rows_cap = 0;

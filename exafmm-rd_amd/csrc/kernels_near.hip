@@ -91,6 +91,7 @@ __global__ __launch_bounds__(64) void k_p2p_f64(const int32_t* __restrict__ leaf
                 const int oi = m_oi[k], cnt = min(64, m_ns[k] - k0);
                 const double xo = xt - (oi % 3 - 1) * cycle, yo = yt - ((oi / 3) % 3 - 1) * cycle,
                              zo = zt - (oi / 9 - 1) * cycle;
+#pragma unroll 4   // independent pairs in flight; the sums keep their order
                 for (int j = g; j < cnt; j += G) {
                     const double4 sv = sbuf[j];
                     const double dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf
                 const int cnt = min(64, m_ns[k] - k0);
                 const float xo = tv.x - m_dx[k], yo = tv.y - m_dy[k], zo = tv.z - m_dz[k];
                 float fp = 0, fx = 0, fy = 0, fz = 0;
+#pragma unroll 4   // independent pairs in flight; the sums keep their order
                 for (int j = g; j < cnt; j += G) {
                     const float4 sv = sbuf[j];
                     const float dx = xo - sv.x, dy = yo - sv.y, dz = zo - sv.z;
@@ -298,6 +300,7 @@ __global__ __launch_bounds__(64) void k_l2p_eval(int S, const double* __restrict
         const int bi = boff + (active ? i : 0);
         const double xt = b.x[bi], yt = b.y[bi], zt = b.z[bi];
         double ap = 0, ax = 0, ay = 0, az = 0;
+#pragma unroll 4   // independent evaluations in flight; the sums keep their order
         for (int s = g; s < S; s += G) {
             const double dx = xt - ex[s], dy = yt - ey[s], dz = zt - ez[s];
             const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
