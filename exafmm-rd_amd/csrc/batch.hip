@@ -63,6 +63,9 @@ struct fmm3dpp_batch {
     std::string err;
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> build_streams;   // one per build thread
+    std::vector<hipStream_t> group_streams;   // groups 1.. run beside group 0 (fork / join events)
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<std::unique_ptr<GpuStructure>> gs;
     std::vector<int64_t> counts, offsets;   // bodies per system and their offsets in the caller's arrays
@@ -89,6 +92,9 @@ struct fmm3dpp_batch {
         groups.clear();   // device buffers, freed before the streams go away
         gs.clear();
         for (hipStream_t bs : build_streams) (void)hipStreamDestroy(bs);
+        for (hipStream_t gs_ : group_streams) (void)hipStreamDestroy(gs_);
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -446,12 +452,22 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     return FMM3DPP_OK;
 }
 
+// All groups: group 0 on s, the others on their own streams forked from s and joined back into it
+// (graph branches under capture).  The groups share only the read-only charges and write disjoint
+// outputs (each system's own bodies).
 int run_all(fmm3dpp_batch* b, const double* d_q, double* d_phi, double* d_grad, hipStream_t s) {
-    for (auto& gp : b->groups) {
-        Group& g = *gp;
-        int rc = run_group(b, g, d_q, d_phi, d_grad, s);
+    const size_t ng = b->groups.size();
+    if (ng > 1) BHIP(hipEventRecord(b->fork, s));
+    for (size_t i = 1; i < ng; i++) {
+        hipStream_t gs_ = b->group_streams[i - 1];
+        BHIP(hipStreamWaitEvent(gs_, b->fork, 0));
+        int rc = run_group(b, *b->groups[i], d_q, d_phi, d_grad, gs_);
         if (rc) return rc;
+        BHIP(hipEventRecord(b->joins[i - 1], gs_));
     }
+    int rc = run_group(b, *b->groups[0], d_q, d_phi, d_grad, s);
+    if (rc) return rc;
+    for (size_t i = 1; i < ng; i++) BHIP(hipStreamWaitEvent(s, b->joins[i - 1], 0));
     return FMM3DPP_OK;
 }
 
@@ -512,6 +528,16 @@ int fmm3dpp_batch_create(const fmm3dpp_config* cfgs, int32_t nsys, int device, f
             it = by_p.find(cfgs[k].P);
         }
         b->groups[it->second]->sys.push_back(k);
+    }
+    if (b->groups.size() > 1 && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        return FMM3DPP_ERR_HIP;
+    for (size_t i = 1; i < b->groups.size(); i++) {
+        hipStream_t gs_ = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&gs_, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
+        b->group_streams.push_back(gs_);
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return FMM3DPP_ERR_HIP;
+        b->joins.push_back(e);
     }
     *out = b.release();
     return FMM3DPP_OK;
