@@ -377,14 +377,14 @@ void pick_tiles(size_t (*lds)(int, int, int, size_t), int P, size_t esz, int& SG
     FC = 1;
 }
 
-// Up transform tiles: all 8 child slots per workgroup and a column chunk that divides the half
-// spectrum, at most 4 columns and 48 KiB of LDS, else the largest chunk that fits two workgroups
-// per CU.  Measured at P=8 on 4681 parents: float32 SG=8, FC=4 0.265 ms against 0.303 ms for FC=8;
-// float64 keeps SG=8, FC=3 (0.446 ms, the best of the sweep).
+// Up transform tiles: all 8 child slots per workgroup and a column chunk of 2 to 4 columns that
+// divides the half spectrum, within two workgroups' LDS per CU; else the largest chunk that fits.
+// Measured at P=8 on 4681 parents: float32 SG=8, FC=4 0.265 ms against 0.303 ms for FC=8; float64
+// SG=8, FC=2 0.454 ms (FC=3 0.446, FC=1 0.626).
 void pick_tiles_up(int P, size_t esz, int& SG, int& FC) {
     const int Nh = P;
-    for (int fc = std::min(4, Nh); fc >= 1; fc--)
-        if (Nh % fc == 0 && m2l_up_lds(P, 8, fc, esz) <= (size_t)48 * 1024) { SG = 8; FC = fc; return; }
+    for (int fc = std::min(4, Nh); fc >= 2; fc--)
+        if (Nh % fc == 0 && m2l_up_lds(P, 8, fc, esz) <= (size_t)80 * 1024) { SG = 8; FC = fc; return; }
     pick_tiles(m2l_up_lds, P, esz, SG, FC);
 }
 
