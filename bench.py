@@ -73,6 +73,49 @@ def cpu_baseline(x, y, z, q, P, ncrit, images, threads, budget_s):
                       f"{dt:.1f} s", "seconds": round(dt, 3)}
 
 
+def plummer_workload(n, seed):
+    """Plummer sphere of scale radius 0.05 centred at (0.5, 0.5, 0.5), wrapped into the periodic unit
+    cube, charges +-1 in equal numbers by a seeded shuffle; returned in the reference frame (x - 0.5)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = np.clip(rng.random(n), 1e-12, 1.0 - 1e-12)
+    r = 0.05 / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    cz = rng.uniform(-1.0, 1.0, n)
+    ph = rng.uniform(0.0, 2 * np.pi, n)
+    st = np.sqrt(1.0 - cz * cz)
+    pts = np.stack([r * st * np.cos(ph), r * st * np.sin(ph), r * cz], axis=1) + 0.5
+    pts = np.mod(pts, 1.0) - 0.5
+    q = np.ones(n)
+    q[: n // 2] = -1.0
+    rng.shuffle(q)
+    return pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), q
+
+
+def run_clustered(pyfmm3dpp, torch, args):
+    """N = 4,194,304 Plummer bodies, P=8, adaptive tree with ncrit=128, 3 image levels, mixed
+    precision: step time, stage breakdown and stage rates, and the difference to the float64 path."""
+    n = args.clustered_n
+    x, y, z, q = plummer_workload(n, args.seed + 3)
+    dev = torch.device("cuda", 0)
+    d_x, d_y, d_z, d_q = (torch.from_numpy(a).to(dev) for a in (x, y, z, q))
+    sub = argparse.Namespace(**vars(args))
+    sub.n, sub.P, sub.ncrit, sub.images = n, 8, 128, 3
+    rmx = run_path(pyfmm3dpp, torch, sub, pyfmm3dpp.PREC_MIXED, d_x, d_y, d_z, d_q, steps=3, warmup=1)
+    r64 = run_path(pyfmm3dpp, torch, sub, pyfmm3dpp.PREC_FP64, d_x, d_y, d_z, d_q, steps=1, warmup=1)
+    c = rmx["plan"].counts()
+    ints, _ = rmx["plan"].cells()
+    leafn = ints[: c["n_tree_cells"], 6][ints[: c["n_tree_cells"], 4] == 0]
+    return {"workload": f"Plummer a=0.05 wrapped, N={n}, +-1 charges, P=8, ncrit=128, images=3, mixed",
+            "particles_per_s": round(n / (rmx["ms_per_step"] / 1e3), 1), "ms_per_step": round(rmx["ms_per_step"], 3),
+            "fp64_ms_per_step": round(r64["ms_per_step"], 3),
+            "tree": {"cells": int(c["n_tree_cells"]), "leaves": int(len(leafn)),
+                     "max_depth": int(ints[: c["n_tree_cells"], 2].max()),
+                     "leaf_bodies_min_mean_max": [int(leafn.min()), round(float(leafn.mean()), 1), int(leafn.max())]},
+            "breakdown_ms": rmx["stages"],
+            "stage_rates": stage_rooflines(rmx["plan"], rmx["stages"], True),
+            "phi_rel_l2_mixed_vs_fp64": float("%.3e" % rel_l2(torch, rmx["phi"], r64["phi"])),
+            "grad_rel_l2_mixed_vs_fp64": float("%.3e" % rel_l2(torch, rmx["grad"], r64["grad"]))}
+
+
 def batch_workload(nsys, nmin, nmax, seed):
     """nsys systems, N log-uniform in [nmin, nmax], P alternating 6 / 8, zero-sum charges."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -153,6 +196,50 @@ def run_batch(pyfmm3dpp, torch, args):
             "sequential_ms_per_step": round(out["sequential"] * 1e3, 3),
             "batch_set_particles_s": round(set_s, 3),
             "rel_l2_batch_vs_sequential": float("%.3e" % diff)}
+
+
+def stage_rooflines(plan, stages, mixed):
+    """Achieved rates of each stage from its algorithmic work (counted from the plan's own tree and
+    lists) and its HIP-event time: flop rates against the matrix-core peak for the operator applies
+    (GEMM flops over the whole stage time, so P2M also pays its check potentials and L2P its field
+    evaluation) and the Hadamard, bytes for the transforms and the near field."""
+    ints, dbl = plan.cells()
+    c = plan.counts()
+    ntree = c["n_tree_cells"]
+    cnum, bnum, depth = ints[:ntree, 4], ints[:ntree, 6], ints[:ntree, 2]
+    leaf = cnum == 0
+    nleaf, nparent = int(leaf.sum()), int((~leaf).sum())
+    S, F = c["S"], c["n_freq"]
+    tar, src, _ = plan.p2p_list()
+    bn = ints[:, 6].astype(np.int64)
+    pairs = float(np.sum(bn[tar] * bn[src]))
+    flops_op = 2.0 * S * S
+    gemm = {"P2M": 2 * flops_op * nleaf, "L2P": 2 * flops_op * nleaf,
+            "M2M": flops_op * (ntree - 1), "L2L": flops_op * (ntree - 1)}
+    out = {}
+    for k, fl in gemm.items():
+        t = stages[k] * 1e-3
+        out[k] = {"gemm_gflop": round(fl / 1e9, 2), "tflops": round(fl / t / 1e12, 2),
+                  "frac_fp64_mfma": round(fl / t / 1e12 / FP64_PEAK_TFLOPS, 3)}
+    had, npairs = hadamard_flops(plan)
+    t = stages["M2L_hadamard"] * 1e-3
+    peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    out["M2L_hadamard"] = {"gflop": round(had / 1e9, 2), "tflops": round(had / t / 1e12, 2),
+                           "frac_mfma": round(had / t / 1e12 / peak, 3)}
+    csz = 8 if mixed else 16   # bytes per complex value of the spectra
+    spec = float(nparent) * F * 8 * csz
+    for k in ("M2L_up", "M2L_down"):
+        t = stages[k] * 1e-3
+        out[k] = {"spectrum_gb": round(spec / 1e9, 3), "gbs": round(spec / t / 1e9, 1),
+                  "frac_hbm": round(spec / t / 1e9 / HBM_PEAK_GBS, 3)}
+    n = int(bnum[0])
+    w = 4 if mixed else 8   # bytes per word: targets 4 words in and 4 out, sources 4 words per row
+    p2p_bytes = n * 8.0 * w + float(np.sum(bn[src])) * 4 * w
+    t = stages["P2P"] * 1e-3
+    out["P2P"] = {"pairs": pairs, "gpairs_per_s": round(pairs / t / 1e9, 1),
+                  "gbs_algorithmic": round(p2p_bytes / t / 1e9, 1),
+                  "frac_hbm": round(p2p_bytes / t / 1e9 / HBM_PEAK_GBS, 3)}
+    return out
 
 
 def profiled_traffic(path):
@@ -236,6 +323,9 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=48,
                     help="systems of the batch leg (N log-uniform 4096..262144, P 6/8, images 2, mixed)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--clustered-n", type=int, default=1 << 22,
+                    help="bodies of the clustered leg (Plummer sphere, adaptive tree, ncrit 128)")
+    ap.add_argument("--no-clustered", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain copy kernel of 1 GiB read + 1 GiB written (elementwise x * 1.0), "
                          "the reference point for the FETCH_SIZE / WRITE_SIZE counters")
@@ -346,8 +436,13 @@ def main():
         "breakdown_ms": {"mixed": rmx["stages"], "fp64": r64["stages"]},
         "precompute_s": round(main_run["precompute_s"], 3),
         "roofline": roofline,
+        "stage_rates": {"mixed": stage_rooflines(rmx["plan"], rmx["stages"], True),
+                        "fp64": stage_rooflines(r64["plan"], r64["stages"], False)},
     }
     log("accuracy legs done")
+    if not args.no_clustered:
+        out["clustered"] = run_clustered(pyfmm3dpp, torch, args)
+        log(f"clustered leg: {out['clustered']['ms_per_step']} ms/step")
     if not args.no_batch:
         out["batch"] = run_batch(pyfmm3dpp, torch, args)
         log(f"batch leg: {out['batch']['speedup']}x over one plan per system")
