@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--clustered-n", type=int, default=1 << 22,
                     help="bodies of the clustered leg (Plummer sphere, adaptive tree, ncrit 128)")
     ap.add_argument("--no-clustered", action="store_true")
+    ap.add_argument("--workload", choices=["uniform", "plummer"], default="uniform",
+                    help="inputs of --single-path runs: the uniform N=--n workload or the clustered leg's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain copy kernel of 1 GiB read + 1 GiB written (elementwise x * 1.0), "
                          "the reference point for the FETCH_SIZE / WRITE_SIZE counters")
@@ -338,7 +340,11 @@ def main():
     import torch
     import pyfmm3dpp
 
-    x, y, z, q = workload(args.n, args.seed)
+    if args.workload == "plummer":   # the clustered leg's inputs, for profiles of that workload
+        args.n, args.ncrit = args.clustered_n, 128
+        x, y, z, q = plummer_workload(args.n, args.seed + 3)
+    else:
+        x, y, z, q = workload(args.n, args.seed)
     dev = torch.device("cuda", 0)
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "device.h"
 
@@ -377,6 +378,19 @@ void pick_tiles(size_t (*lds)(int, int, int, size_t), int P, size_t esz, int& SG
     FC = 1;
 }
 
+// Raises a kernel's dynamic LDS limit to 160 KiB once per device (the attribute is per device;
+// `done` holds one bit per device id, safe to call from several host threads).
+hipError_t allow_large_lds(const void* fn, std::atomic<uint32_t>& done) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint32_t bit = dev < 32 ? 1u << dev : 0u;
+    if (bit && (done.load(std::memory_order_acquire) & bit)) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess && bit) done.fetch_or(bit, std::memory_order_release);
+    return e;
+}
+
 // Up transform tiles: all 8 child slots per workgroup and a column chunk of 2 to 4 columns that
 // divides the half spectrum, within two workgroups' LDS per CU; else the largest chunk that fits.
 // Measured at P=8 on 4681 parents: float32 SG=8, FC=4 0.265 ms against 0.303 ms for FC=8; float64
@@ -394,13 +408,9 @@ hipError_t up_p(int S, int nsrc, const int32_t* src_child, const int32_t* inv_la
     int SG, FC;
     pick_tiles_up(P, sizeof(T), SG, FC);
     const size_t lds = m2l_up_lds(P, SG, FC, sizeof(T));
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_m2l_up<P, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static std::atomic<uint32_t> attr{0};
+    hipError_t e = allow_large_lds((const void*)k_m2l_up<P, T>, attr);
+    if (e != hipSuccess) return e;
     k_m2l_up<P, T><<<nsrc * (8 / SG), 256, lds, st>>>(S, SG, FC, src_child, inv_lat, twiddle, qeq,
                                                        static_cast<cplx<T>*>(Qk));
     return hipGetLastError();
@@ -424,13 +434,9 @@ hipError_t down_p(int S, int ntar, const int32_t* tar_child, const double* tar_s
     int SG, FC;
     pick_tiles_down(P, sizeof(T), SG, FC);
     const size_t lds = m2l_down_lds(P, SG, FC, sizeof(T));
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_m2l_down<P, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static std::atomic<uint32_t> attr{0};
+    hipError_t e = allow_large_lds((const void*)k_m2l_down<P, T>, attr);
+    if (e != hipSuccess) return e;
     k_m2l_down<P, T><<<ntar * (8 / SG), 256, lds, st>>>(S, SG, FC, tar_child, tar_scale, inv_lat, twiddle,
                                                          static_cast<const cplx<T>*>(Pk), pck);
     return hipGetLastError();
