@@ -51,6 +51,7 @@ struct Group {
     DBuf x, y, z, perm, cx, cy, cz, cr, depth, boff, bnum, leaf_cell, leaf_depth, iota, p2p_off, p2p_ent, m2m_in,
         m2m_out, l2l_in, l2l_out, img_a, img_b, child_slots, tar_scale, slot_src_T, img_src_T, img_tar, seg_off;
     DeviceOperators dop;
+    M2LTiles up_tiles, down_tiles;   // transform tiles of the tree parents (0, 0: measured defaults)
     // work buffers
     DBuf q, pot, gx, gy, gz, pcleaf, pcleaf2, qeq, pck, Qk, Pk, Qk_img, Pk_img, rel, m2m_part, partial, moments;
 };
@@ -391,15 +392,17 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     const int32_t* inv = g.dop.o_invlat.as<int32_t>();
     const double* tw = g.dop.o_tw.as<double>();
     if (!mixed) {
-        BHIP(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, g.Qk.p, s));
+        BHIP(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, g.Qk.p, s, g.up_tiles));
         BHIP(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.dop.o_gk8.p, g.Qk.p, g.Pk.p, s));
         BHIP(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                g.dop.o_gk27.as<double>(), g.Qk.as<double>(), g.Pk.as<double>(), s));
-        BHIP(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s));
+        BHIP(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s,
+                             g.down_tiles));
     } else {
-        BHIP(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, g.Qk.p, s));
+        BHIP(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, g.Qk.p, s, g.up_tiles));
         BHIP(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.dop.o_gk8f.p, g.Qk.p, g.Pk.p, s));
-        BHIP(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s));
+        BHIP(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s,
+                             g.down_tiles));
         if (nimg > 0) {
             BHIP(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, g.Qk_img.p, s));
             BHIP(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, nl,
@@ -484,6 +487,46 @@ int ensure_operators(fmm3dpp_batch* b) {
             int rc = upload_group_operators(b, g);
             if (rc) return rc;
         }
+    }
+    return FMM3DPP_OK;
+}
+
+// Time of one evaluation of group g (the second of two runs), milliseconds.
+int time_group(fmm3dpp_batch* b, Group& g, const double* d_q, double* d_phi, double* d_grad, float& ms) {
+    hipStream_t s = b->stream;
+    for (int rep = 0; rep < 2; rep++) {
+        BHIP(hipEventRecord(b->ev0, s));
+        int rc = run_group(b, g, d_q, d_phi, d_grad, s);
+        if (rc) return rc;
+        BHIP(hipEventRecord(b->ev1, s));
+        BHIP(hipEventSynchronize(b->ev1));
+    }
+    BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    return FMM3DPP_OK;
+}
+
+// Transform tiles of the group's tree parents: each candidate (slot group, column chunk) of the down
+// and then the up transform is timed over a whole group evaluation; the fastest is kept.  The
+// results do not depend on the tiles.
+int tune_tiles(fmm3dpp_batch* b, Group& g, const double* d_q, double* d_phi, double* d_grad) {
+    const bool f32 = b->mixed();
+    const M2LTiles down_cand[] = {{0, 0}, {4, 4}, {8, 2}, {4, 2}, {8, 4}, {2, 8}};
+    const M2LTiles up_cand[] = {{0, 0}, {8, 4}, {8, 2}, {4, 4}, {8, 3}};
+    for (int pass = 0; pass < 2; pass++) {
+        M2LTiles& cur = pass == 0 ? g.down_tiles : g.up_tiles;
+        const M2LTiles* cand = pass == 0 ? down_cand : up_cand;
+        const int ncand = pass == 0 ? 6 : 5;
+        M2LTiles best = cur;
+        float best_ms = 1e30f;
+        for (int c = 0; c < ncand; c++) {
+            if (!m2l_tiles_valid(g.P, f32, pass == 0, cand[c])) continue;
+            cur = cand[c];
+            float ms = 0;
+            int rc = time_group(b, g, d_q, d_phi, d_grad, ms);
+            if (rc) return rc;
+            if (ms < best_ms) { best_ms = ms; best = cand[c]; }
+        }
+        cur = best;
     }
     return FMM3DPP_OK;
 }
@@ -658,20 +701,16 @@ int fmm3dpp_batch_autotune(fmm3dpp_batch* b, const int32_t* candidates, int32_t 
             rc = build_group(b, g);
             if (rc) return rc;
             const double tb = now_ms() - t0;
-            for (int rep = 0; rep < 2; rep++) {   // the second evaluation is timed
-                BHIP(hipEventRecord(b->ev0, s));
-                rc = run_group(b, g, d_q, d_phi, d_grad, s);
-                if (rc) return rc;
-                BHIP(hipEventRecord(b->ev1, s));
-                BHIP(hipEventSynchronize(b->ev1));
-            }
             float ms = 0;
-            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            rc = time_group(b, g, d_q, d_phi, d_grad, ms);
+            if (rc) return rc;
             const double t = ms + tb;   // one step = tree and lists + evaluation
             if (t < best) { best = t; best_c = candidates[c]; }
         }
         for (int k : g.sys) b->cfg[k].ncrit = best_c;
         rc = build_group(b, g);
+        if (rc) return rc;
+        rc = tune_tiles(b, g, d_q, d_phi, d_grad);
         if (rc) return rc;
     }
     if (chosen)

@@ -101,7 +101,7 @@ __global__ void k_keys(int64_t n, const double* __restrict__ x, const double* __
 // is log2(bodies) rather than 8 x that.  cstart[9 * lc + o] = first body of octant o, [8] = end.
 __global__ void k_split(int ls, int le, int l, int ncrit, const uint64_t* __restrict__ K1,
                         const int32_t* __restrict__ boff, const int32_t* __restrict__ bnum, int32_t* __restrict__ nchild,
-                        int32_t* __restrict__ cstart, int* __restrict__ flag) {
+                        int32_t* __restrict__ cstart, int* __restrict__ flag, int32_t* __restrict__ lvl_nl) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int o = (int)(gt & 7);
     const int64_t c64 = ls + (gt >> 3);
@@ -131,6 +131,7 @@ __global__ void k_split(int ls, int le, int l, int ncrit, const uint64_t* __rest
         if (o == 7) cstart[lc * 9 + 8] = b + m;
     }
     if (o == 0) nchild[lc] = split ? nonempty : 0;
+    if (o == 0 && split) atomicAdd(&lvl_nl[l], 1);   // non-leaf cells of level l (a count: order-free)
 }
 
 __global__ void k_emit_children(int ls, int le, const int32_t* __restrict__ nchild, const int32_t* __restrict__ incl,
@@ -214,33 +215,36 @@ __global__ void k_iota(int n, int32_t* __restrict__ a) {
 
 // Column tables of the per-level operator applies: M2M reads the 8 children of a parent column
 // (octant-major) and writes the parent; L2L reads the parent and writes each child.
-__global__ void k_level_tables(int ntree, int maxd, const int32_t* __restrict__ info /* ls[0..maxd], tab_off, col_off */,
+// Operator-apply tables of the tree levels, one column per non-leaf cell of the level (leaves have
+// no children to gather or scatter).  info = [ls | tab_off | col_off | nls], each [0..maxd]: level
+// starts, table offsets, column offsets, and the number of non-leaf cells before each level;
+// nl_incl is the inclusive scan of the non-leaf flags.
+__global__ void k_level_tables(int ntree, int maxd, const int32_t* __restrict__ info,
                                const int32_t* __restrict__ depth, const int32_t* __restrict__ coff,
                                const int32_t* __restrict__ cnum, const int32_t* __restrict__ octant,
-                               int32_t* __restrict__ m2m_in, int32_t* __restrict__ m2m_out,
-                               int32_t* __restrict__ l2l_in, int32_t* __restrict__ l2l_out) {
+                               const int32_t* __restrict__ nl_incl, int32_t* __restrict__ m2m_in,
+                               int32_t* __restrict__ m2m_out, int32_t* __restrict__ l2l_in,
+                               int32_t* __restrict__ l2l_out) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ntree) return;
-    int l = depth[c];
-    if (l >= maxd) return;
-    const int32_t* ls = info;
+    const int nc = cnum[c], l = depth[c];
+    if (nc == 0 || l >= maxd) return;
     const int32_t* tab_off = info + (maxd + 1);
     const int32_t* col_off = info + 2 * (maxd + 1);
-    int col = c - ls[l], ncol = ls[l + 1] - ls[l];
-    int64_t tab = tab_off[l];
+    const int32_t* nls = info + 3 * (maxd + 1);
+    const int col = nl_incl[c] - 1 - nls[l], ncol = nls[l + 1] - nls[l];
+    const int64_t tab = tab_off[l];
     for (int o = 0; o < 8; o++) {
         m2m_in[tab + (int64_t)o * ncol + col] = -1;
         l2l_out[tab + (int64_t)o * ncol + col] = -1;
         l2l_in[tab + (int64_t)o * ncol + col] = c;
     }
-    int nc = cnum[c];
     for (int k = 0; k < nc; k++) {
-        int ch = coff[c] + k;
-        int o = octant[ch];
+        const int ch = coff[c] + k, o = octant[ch];
         m2m_in[tab + (int64_t)o * ncol + col] = ch;
         l2l_out[tab + (int64_t)o * ncol + col] = ch;
     }
-    m2m_out[col_off[l] + col] = nc > 0 ? c : -1;
+    m2m_out[col_off[l] + col] = c;
 }
 
 __device__ inline void deinterleave_d(uint64_t prefix, int depth, int64_t c[3]) {
@@ -559,13 +563,16 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
                                  depth.as<int32_t>(), octant.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
                                  coff.as<int32_t>(), cnum.as<int32_t>());
     CK(hipGetLastError());
+    CK(lvl_nl.ensure((kD + 2) * 4));
+    CK(hipMemsetAsync(lvl_nl.p, 0, (kD + 2) * 4, st));
     level_start.assign(1, 0);
     level_start.push_back(1);
     for (int l = 0;; l++) {
         int ls = level_start[l], le = level_start[l + 1], ncl = le - ls;
         CK(nchild.ensure((size_t)ncl * 4)); CK(scan32.ensure((size_t)ncl * 4)); CK(cstart.ensure((size_t)ncl * 9 * 4));
         k_split<<<blocks((int64_t)ncl * 8), 256, 0, st>>>(ls, le, l, cfg.ncrit, key_s.as<uint64_t>(), boff.as<int32_t>(),
-                                              bnum.as<int32_t>(), nchild.as<int32_t>(), cstart.as<int32_t>(), flag);
+                                              bnum.as<int32_t>(), nchild.as<int32_t>(), cstart.as<int32_t>(), flag,
+                                              lvl_nl.as<int32_t>());
         CK(hipGetLastError());
         CK(scan_incl(temp, nchild.as<int32_t>(), scan32.as<int32_t>(), ncl, st));
         k_copy_last<<<1, 1, 0, st>>>(scan32.as<int32_t>(), ncl - 1, flag);
@@ -626,14 +633,26 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
                                        cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(), prefix.as<uint64_t>());
         CK(hipGetLastError());
     }
-    // --- 4. per-level operator-apply tables; lvl_info = [ls[0..maxd] | tab_off[0..maxd] | col_off[0..maxd]]
-    std::vector<int32_t> info(3 * (maxd + 1), 0);
+    // --- 4. per-level operator-apply tables over the non-leaf cells of each level;
+    //        lvl_info = [ls | tab_off | col_off | nls], each [0..maxd]
+    CK(flags.ensure((size_t)ntree * 4)); CK(nl_idx.ensure((size_t)ntree * 4));
+    k_nonleaf_flag<<<blocks(ntree), 256, 0, st>>>(ntree, cnum.as<int32_t>(), flags.as<int32_t>());
+    CK(hipGetLastError());
+    CK(scan_incl(temp, flags.as<int32_t>(), nl_idx.as<int32_t>(), ntree, st));
+    std::vector<int32_t> lvl_count(maxd + 1, 0);
+    CK(read_back(lvl_nl.as<int32_t>(), lvl_count.data(), (size_t)maxd + 1, st));
+    std::vector<int32_t> info(4 * (maxd + 1), 0);
     m2m_levels.clear(); l2l_levels.clear();
     max_level_cols = 1;
     int64_t tab = 0, col = 0;
-    for (int l = 0; l <= maxd; l++) info[l] = level_start[l];
+    int32_t nls = 0;
+    for (int l = 0; l <= maxd; l++) {
+        info[l] = level_start[l];
+        info[3 * (maxd + 1) + l] = nls;
+        nls += lvl_count[l];
+    }
     for (int l = 0; l < maxd; l++) {
-        int ncol = level_start[l + 1] - level_start[l];
+        const int ncol = lvl_count[l];
         info[(maxd + 1) + l] = (int32_t)tab;
         info[2 * (maxd + 1) + l] = (int32_t)col;
         l2l_levels.push_back({ncol, tab, col});
@@ -648,7 +667,8 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     if (maxd > 0) {
         k_level_tables<<<blocks(ntree), 256, 0, st>>>(ntree, maxd, lvl_info.as<int32_t>(), depth.as<int32_t>(),
                                                        coff.as<int32_t>(), cnum.as<int32_t>(), octant.as<int32_t>(),
-                                                       m2m_in.as<int32_t>(), m2m_out.as<int32_t>(), l2l_in.as<int32_t>(),
+                                                       nl_idx.as<int32_t>(), m2m_in.as<int32_t>(),
+                                                       m2m_out.as<int32_t>(), l2l_in.as<int32_t>(),
                                                        l2l_out.as<int32_t>());
         CK(hipGetLastError());
     }
@@ -666,14 +686,8 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
         for (auto& p : img_l2l) { ia.push_back(p[0]); ib.push_back(p[1]); }
         CK(upload(img_a, ia, st)); CK(upload(img_b, ib, st));
     }
-    // --- 5. M2L tables
-    CK(flags.ensure((size_t)ntree * 4)); CK(nl_idx.ensure((size_t)ntree * 4));
-    k_nonleaf_flag<<<blocks(ntree), 256, 0, st>>>(ntree, cnum.as<int32_t>(), flags.as<int32_t>());
-    CK(hipGetLastError());
-    CK(scan_incl(temp, flags.as<int32_t>(), nl_idx.as<int32_t>(), ntree, st));
-    int32_t nlc = 0;
-    CK(read_back(nl_idx.as<int32_t>() + (ntree - 1), &nlc, 1, st));
-    nl = nlc;
+    // --- 5. M2L tables (targets = sources = the non-leaf cells, counted per level above)
+    nl = nls;
     nimg = std::max(0, images - 1);
     ntar = nl + nimg;
     CK(nl_cells.ensure((size_t)std::max(ntar, 1) * 4)); CK(child_slots.ensure((size_t)std::max(ntar, 1) * 32));

@@ -57,8 +57,11 @@ hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, cons
 // M2L (f32 selects the float32 complex layout of Qk/Pk and gk8; the transforms read q_equiv and
 // accumulate p_check in float64 either way).  Qk/Pk are [parent][f][8 children] complex;
 // inv_lat maps the P^3 corner of the convolution grid ((k * P + j) * P + i) to a surface point or -1.
+// Tiles: SG child slots per workgroup (1, 2, 4 or 8) and FC half-spectrum columns per LDS chunk
+// (1..P); 0, 0 chooses them from the measured defaults.  Results do not depend on the tiles.
+struct M2LTiles { int sg = 0, fc = 0; };
 hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
-                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st);
+                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st, M2LTiles tiles = {});
 // targets t < ntar, slot_src_T rows of stride ntab (-1: no source)
 hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
                             const void* Qk, void* Pk, hipStream_t st);
@@ -67,7 +70,9 @@ hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int3
                              const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);
 hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
                            const int32_t* inv_lat, const double* twiddle, const void* Pk, double* pck,
-                           hipStream_t st);
+                           hipStream_t st, M2LTiles tiles = {});
+// Whether explicit tiles fit (slot group divides 8, chunk within the half spectrum, LDS <= 160 KiB).
+bool m2l_tiles_valid(int P, bool f32, bool down, M2LTiles tiles);
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
                            hipStream_t st);

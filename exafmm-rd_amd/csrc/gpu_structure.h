@@ -39,7 +39,7 @@ struct GpuStructure {
     DBuf rows_tc, rows_sc, rows_oi, p2p_off, p2p_ent;
     size_t rows_cap = 0;
     // temporaries
-    DBuf key, key_s, idx0, perm1, tkey, tkey_s, ldep, head, rank, nchild, cstart, scan32, packed, scan64,
+    DBuf key, key_s, idx0, perm1, tkey, tkey_s, ldep, head, rank, nchild, cstart, scan32, packed, scan64, lvl_nl,
         ekey, eval, ekey_s, eval_s, cnt64, flags, lvl_info, small, temp;
     DBuf f_tc[2], f_sc[2], f_tcp[2], f_scp[2], f_oi[2];
     size_t f_cap[2] = {0, 0};

@@ -404,9 +404,9 @@ void pick_tiles_up(int P, size_t esz, int& SG, int& FC) {
 
 template <int P, typename T>
 hipError_t up_p(int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat, const double* twiddle,
-                const double* qeq, void* Qk, hipStream_t st) {
-    int SG, FC;
-    pick_tiles_up(P, sizeof(T), SG, FC);
+                const double* qeq, void* Qk, hipStream_t st, M2LTiles tiles) {
+    int SG = tiles.sg, FC = tiles.fc;
+    if (SG == 0 || FC == 0) pick_tiles_up(P, sizeof(T), SG, FC);
     const size_t lds = m2l_up_lds(P, SG, FC, sizeof(T));
     static std::atomic<uint32_t> attr{0};
     hipError_t e = allow_large_lds((const void*)k_m2l_up<P, T>, attr);
@@ -430,9 +430,9 @@ void pick_tiles_down(int P, size_t esz, int& SG, int& FC) {
 
 template <int P, typename T>
 hipError_t down_p(int S, int ntar, const int32_t* tar_child, const double* tar_scale, const int32_t* inv_lat,
-                  const double* twiddle, const void* Pk, double* pck, hipStream_t st) {
-    int SG, FC;
-    pick_tiles_down(P, sizeof(T), SG, FC);
+                  const double* twiddle, const void* Pk, double* pck, hipStream_t st, M2LTiles tiles) {
+    int SG = tiles.sg, FC = tiles.fc;
+    if (SG == 0 || FC == 0) pick_tiles_down(P, sizeof(T), SG, FC);
     const size_t lds = m2l_down_lds(P, SG, FC, sizeof(T));
     static std::atomic<uint32_t> attr{0};
     hipError_t e = allow_large_lds((const void*)k_m2l_down<P, T>, attr);
@@ -460,15 +460,23 @@ hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, con
 
 }  // namespace
 
+bool m2l_tiles_valid(int P, bool f32, bool down, M2LTiles t) {
+    if (t.sg == 0 && t.fc == 0) return true;
+    if ((t.sg != 1 && t.sg != 2 && t.sg != 4 && t.sg != 8) || t.fc < 1 || t.fc > P) return false;
+    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    return (down ? m2l_down_lds(P, t.sg, t.fc, esz) : m2l_up_lds(P, t.sg, t.fc, esz)) <= (size_t)160 * 1024;
+}
+
 hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
-                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st) {
+                         const double* twiddle, const double* qeq, void* Qk, hipStream_t st, M2LTiles tiles) {
     if (nsrc <= 0) return hipSuccess;
     if (!src_child || !inv_lat || !twiddle || !qeq || !Qk) return hipErrorInvalidValue;
+    if (!m2l_tiles_valid(P, f32, false, tiles)) return hipErrorInvalidValue;
     switch (P) {
 #define UPCASE(p)                                                                                  \
     case p:                                                                                        \
-        return f32 ? up_p<p, float>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st)             \
-                   : up_p<p, double>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st);
+        return f32 ? up_p<p, float>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st, tiles)      \
+                   : up_p<p, double>(S, nsrc, src_child, inv_lat, twiddle, qeq, Qk, st, tiles);
         FMM3DPP_P_CASES(UPCASE)
 #undef UPCASE
         default: return hipErrorInvalidValue;
@@ -495,14 +503,15 @@ hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int3
 
 hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_child, const double* tar_scale,
                            const int32_t* inv_lat, const double* twiddle, const void* Pk, double* pck,
-                           hipStream_t st) {
+                           hipStream_t st, M2LTiles tiles) {
     if (ntar <= 0) return hipSuccess;
     if (!tar_child || !tar_scale || !inv_lat || !twiddle || !Pk || !pck) return hipErrorInvalidValue;
+    if (!m2l_tiles_valid(P, f32, true, tiles)) return hipErrorInvalidValue;
     switch (P) {
 #define DNCASE(p)                                                                                          \
     case p:                                                                                                \
-        return f32 ? down_p<p, float>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st)        \
-                   : down_p<p, double>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st);
+        return f32 ? down_p<p, float>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st, tiles) \
+                   : down_p<p, double>(S, ntar, tar_child, tar_scale, inv_lat, twiddle, Pk, pck, st, tiles);
         FMM3DPP_P_CASES(DNCASE)
 #undef DNCASE
         default: return hipErrorInvalidValue;

@@ -53,7 +53,7 @@ extern "C" {
                                   /* evaluate returns FMM3DPP_ERR_DEVICE                          */
 
 typedef struct fmm3dpp_config {
-    int P;              /* points per surface edge, 2..16  (Argument::P, -P)                 */
+    int P;              /* points per surface edge, 2..12  (Argument::P, -P)                 */
     int ncrit;          /* max bodies per leaf, >= 1       (Argument::ncrit, -m)             */
     int images;         /* periodic image levels, 0..20    (Argument::images, -i); 0 = free space */
     double cycle;       /* period (box edge)               (Argument::cycle, -c)             */
@@ -193,7 +193,9 @@ int fmm3dpp_batch_evaluate_device(fmm3dpp_batch* batch, const double* d_q, doubl
 int fmm3dpp_batch_set_graph(fmm3dpp_batch* batch, int enable);
 /* Leaf size per P group: for each candidate ncrit, rebuilds the group's trees and lists and times one
  * evaluation with charges d_q (outputs land in d_phi, d_grad); keeps the fastest (tree build +
- * evaluation).  chosen[nsys] (may be NULL) receives the ncrit of every system. */
+ * evaluation).  Then the M2L transform tiles (child slots per workgroup, spectrum columns per LDS
+ * chunk) of each group are timed over a few candidates and the fastest kept; results do not depend
+ * on the tiles.  chosen[nsys] (may be NULL) receives the ncrit of every system. */
 int fmm3dpp_batch_autotune(fmm3dpp_batch* batch, const int32_t* candidates, int32_t ncand, const double* d_q,
                            double* d_phi, double* d_grad, int32_t* chosen);
 /* Milliseconds of the last evaluate ("evaluate", HIP events), set_particles ("set_particles", host
