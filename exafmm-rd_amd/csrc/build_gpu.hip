@@ -280,23 +280,32 @@ __global__ void k_m2l_tables(int ntree, int ntar, int periodic, const int32_t* _
                              int32_t* __restrict__ nl_cells, int32_t* __restrict__ child_slots,
                              double* __restrict__ tar_scale, int32_t* __restrict__ slot_src_T,
                              int32_t* __restrict__ img_src_T) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ntree || cnum[c] == 0) return;
+    // one thread per (cell, item): items 0..25 are the neighbour patterns, item 26 the cell's own
+    // entries (target index, child slots, scale)
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c64 = gt / 27;
+    const int p = (int)(gt - c64 * 27);
+    if (c64 >= ntree) return;
+    const int c = (int)c64;
+    if (cnum[c] == 0) return;
     const int32_t* ls = info;
-    int t = nl_incl[c] - 1;
-    nl_cells[t] = c;
-    for (int o = 0; o < 8; o++) child_slots[t * 8 + o] = -1;
-    for (int k = 0; k < cnum[c]; k++) {
-        int ch = coff[c] + k;
-        child_slots[t * 8 + octant[ch]] = ch;
-    }
+    const int t = nl_incl[c] - 1;
     const int d = depth[c];
-    tar_scale[t] = ldexp(1.0, d + 1);
+    if (p == 26) {
+        nl_cells[t] = c;
+        for (int o = 0; o < 8; o++) child_slots[t * 8 + o] = -1;
+        for (int k = 0; k < cnum[c]; k++) {
+            const int ch = coff[c] + k;
+            child_slots[t * 8 + octant[ch]] = ch;
+        }
+        tar_scale[t] = ldexp(1.0, d + 1);
+        return;
+    }
     int64_t a[3];
     deinterleave_d(prefix[c], d, a);
     const int64_t ns = int64_t(1) << d;
     const int lo0 = ls[d], hi0 = ls[d + 1];
-    for (int p = 0; p < 26; p++) {
+    {
         int lin = p < 13 ? p : p + 1;
         int dl[3] = {lin % 3 - 1, (lin / 3) % 3 - 1, lin / 9 - 1};
         int64_t b[3];
@@ -494,13 +503,18 @@ hipError_t scan_incl(DBuf& temp, const T* in, T* out, int64_t n, hipStream_t st)
     return rocprim::inclusive_scan(temp.p, bytes, in, out, (size_t)n, rocprim::plus<T>(), st);
 }
 
+// Stable LSD radix sort over the low `bits` bits.  The onesweep algorithm is used at every size
+// above one block (rocPRIM's default switches to a merge sort up to 2^20 items, measured ~2x
+// slower here for the body keys and the list entries).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
 template <class K, class V>
 hipError_t sort_pairs(DBuf& temp, const K* kin, K* kout, const V* vin, V* vout, int64_t n, int bits, hipStream_t st) {
     size_t bytes = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    hipError_t e = rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
     if (e != hipSuccess) return e;
     if ((e = temp.ensure(bytes)) != hipSuccess) return e;
-    return rocprim::radix_sort_pairs(temp.p, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    return rocprim::radix_sort_pairs<SortConfig>(temp.p, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
 }
 
 template <class T>
@@ -605,7 +619,9 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     CK(hipGetLastError());
     CK(perm.ensure(n * 4));
     if (cfg.ncrit <= kLeafSortMax) {   // leaves hold <= ncrit bodies: sort each leaf's slice by input index
-        k_leaf_sort<<<ntree, 256, 0, st>>>(ntree, cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
+        int lsb = 64;   // one wave for leaves of up to 64 bodies, up to 4 for larger ncrit
+        while (lsb < cfg.ncrit && lsb < 256) lsb *= 2;
+        k_leaf_sort<<<ntree, lsb, 0, st>>>(ntree, cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
                                            perm1.as<int32_t>(), perm.as<int32_t>());
         CK(hipGetLastError());
     } else {                           // stable re-sort by the key truncated at the leaf depth
@@ -694,7 +710,7 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     CK(tar_scale.ensure((size_t)std::max(ntar, 1) * 8));
     CK(slot_src_T.ensure((size_t)26 * std::max(ntar, 1) * 4)); CK(img_src_T.ensure((size_t)26 * std::max(ntar, 1) * 4));
     if (nl > 0) {
-        k_m2l_tables<<<blocks(ntree), 256, 0, st>>>(ntree, ntar, images >= 1 ? 1 : 0, lvl_info.as<int32_t>(),
+        k_m2l_tables<<<blocks((int64_t)ntree * 27), 256, 0, st>>>(ntree, ntar, images >= 1 ? 1 : 0, lvl_info.as<int32_t>(),
                                                      prefix.as<uint64_t>(), depth.as<int32_t>(), coff.as<int32_t>(),
                                                      cnum.as<int32_t>(), octant.as<int32_t>(), nl_idx.as<int32_t>(),
                                                      nl_cells.as<int32_t>(), child_slots.as<int32_t>(),
