@@ -494,8 +494,39 @@ __global__ void k_image_cells(int ntree, int images, int64_t n, double* __restri
     }
 }
 
+// Inclusive scan of up to kSmallScan items in one workgroup: each thread sums a contiguous run, the
+// run totals are scanned in LDS, then each run is rewritten with its offset.  Integer sums, so the
+// result equals any other order's.
+constexpr int kSmallScan = 4096;
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_small(const T* __restrict__ in, T* __restrict__ out, int n) {
+    __shared__ T part[1024];
+    const int tid = threadIdx.x, per = (n + 1023) / 1024;
+    const int b = min(n, tid * per), e = min(n, b + per);
+    T acc = 0;
+    for (int i = b; i < e; i++) acc += in[i];
+    part[tid] = acc;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // Hillis-Steele over the run totals
+        const T v = tid >= d ? part[tid - d] : (T)0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    T run = tid > 0 ? part[tid - 1] : (T)0;
+    for (int i = b; i < e; i++) {
+        run += in[i];
+        out[i] = run;
+    }
+}
+
 template <class T>
 hipError_t scan_incl(DBuf& temp, const T* in, T* out, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (n <= kSmallScan) {
+        k_scan_small<T><<<1, 1024, 0, st>>>(in, out, (int)n);
+        return hipGetLastError();
+    }
     size_t bytes = 0;
     hipError_t e = rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, rocprim::plus<T>(), st);
     if (e != hipSuccess) return e;
@@ -534,7 +565,14 @@ int bits_for(int64_t v) {
 
 int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx, const double* dy, const double* dz,
                         hipStream_t st, std::string& err) {
-    auto t_start = std::chrono::steady_clock::now();
+    int rc = build_tree(cfg, nb, dx, dy, dz, st, err);
+    if (rc) return rc;
+    return build_lists(cfg, st, err);
+}
+
+int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double* dx, const double* dy,
+                             const double* dz, hipStream_t st, std::string& err) {
+    t_build = std::chrono::steady_clock::now();
     if (nb < 1 || nb > (int64_t)INT32_MAX / 2) { err = "number of bodies must be in [1, 2^30)"; return FMM3DPP_ERR_ARG; }
     n = nb;
     const int images = cfg.images;
@@ -731,6 +769,13 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     } else {
         CK(img_tar.ensure(4));
     }
+    return FMM3DPP_OK;
+}
+
+// Steps 6-7, the near-field lists; needs only the tree of build_tree, so it may run on another stream
+// while the far-field stages of an evaluation run on the tree's stream.
+int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::string& err) {
+    const int images = cfg.images;
     // --- 6. P2P map by frontier traversal
     std::vector<int32_t> init_tc, init_sc, init_tcp, init_scp, init_oi;
     const int top = images == 0 ? 0 : ntree;
@@ -824,7 +869,7 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
     k_csr_offsets<<<blocks(nleaf + 1), 256, 0, st>>>(nleaf, n_entries, ekey_s.as<int32_t>(), p2p_off.as<int64_t>());
     CK(hipGetLastError());
     CK(hipStreamSynchronize(st));
-    ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     return FMM3DPP_OK;
 }
 

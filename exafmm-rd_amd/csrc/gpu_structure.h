@@ -1,5 +1,6 @@
 // Tree and interaction lists built on the GPU (bit-exact with the reference's Tree/Traverser).
 #pragma once
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -44,8 +45,14 @@ struct GpuStructure {
     DBuf f_tc[2], f_sc[2], f_tcp[2], f_scp[2], f_oi[2];
     size_t f_cap[2] = {0, 0};
 
+    // build = build_tree (keys, sort, levels, leaves, operator-apply and M2L tables) + build_lists (the
+    // P2P lists); build_lists reads only what build_tree wrote and may use another stream.
     int build(const fmm3dpp_config& cfg, int64_t n, const double* dx, const double* dy, const double* dz,
               hipStream_t st, std::string& err);
+    int build_tree(const fmm3dpp_config& cfg, int64_t n, const double* dx, const double* dy, const double* dz,
+                   hipStream_t st, std::string& err);
+    int build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::string& err);
+    std::chrono::steady_clock::time_point t_build;
     // Host copies for the structure queries of the C-ABI.
     int export_structure(const fmm3dpp_config& cfg, Structure& out, hipStream_t st, std::string& err);
 };

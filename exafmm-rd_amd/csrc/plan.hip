@@ -40,7 +40,8 @@ struct fmm3dpp_plan {
     Structure st;              // host tree/lists (structure-only plans; export cache for device plans)
     GpuStructure gs;           // device tree/lists (device plans)
     std::map<std::string, double> host_ms;
-    hipEvent_t ev[kStages + 1] = {};
+    hipEvent_t ev[kStages + 2] = {};   // stage boundaries; ev[kStages + 1]: end of the far field
+    hipStream_t list_stream = nullptr;   // near-field list build beside the far field (fmm3dpp_solve_device)
     bool ev_recorded = false;
 
     DBuf d_inx, d_iny, d_inz, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
@@ -114,9 +115,12 @@ int ensure_ops(fmm3dpp_plan* plan) {
 }
 
 // Device tree/lists plus the per-evaluate work buffers sized for them.
-int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* dy, const double* dz) {
+// with_lists = false leaves the near-field lists to GpuStructure::build_lists (fmm3dpp_solve_device).
+int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* dy, const double* dz,
+                 bool with_lists = true) {
     hipStream_t s = plan->stream;
-    int rc = plan->gs.build(plan->cfg, n, dx, dy, dz, s, plan->err);
+    int rc = with_lists ? plan->gs.build(plan->cfg, n, dx, dy, dz, s, plan->err)
+                        : plan->gs.build_tree(plan->cfg, n, dx, dy, dz, s, plan->err);
     if (rc) return rc;
     const GpuStructure& g = plan->gs;
     const int Sn = surface_count(plan->cfg.P);
@@ -158,113 +162,9 @@ int ensure_host_structure(fmm3dpp_plan* plan) {
     return FMM3DPP_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** out) {
-    if (!out) return FMM3DPP_ERR_ARG;
-    *out = nullptr;
-    std::string err;
-    int rc = validate_config(cfg, err);
-    if (rc) return rc;
-    std::unique_ptr<fmm3dpp_plan> plan(new fmm3dpp_plan());
-    plan->cfg = *cfg;
-    plan->device = device;
-    if (device != FMM3DPP_DEVICE_NONE) {
-        int count = 0;
-        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FMM3DPP_ERR_DEVICE;
-        if (device < 0 || device >= count) return FMM3DPP_ERR_DEVICE;
-        if (hipSetDevice(device) != hipSuccess) return FMM3DPP_ERR_DEVICE;
-        if (hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
-        plan->own_stream = true;
-        for (int i = 0; i <= kStages; i++)
-            if (hipEventCreate(&plan->ev[i]) != hipSuccess) return FMM3DPP_ERR_HIP;
-    }
-    *out = plan.release();
-    return FMM3DPP_OK;
-}
-
-int fmm3dpp_plan_destroy(fmm3dpp_plan* plan) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (plan->device >= 0) {
-        (void)hipSetDevice(plan->device);
-        if (plan->stream) (void)hipStreamSynchronize(plan->stream);
-        for (int i = 0; i <= kStages; i++) if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
-        if (plan->own_stream && plan->stream) (void)hipStreamDestroy(plan->stream);
-    }
-    delete plan;
-    return FMM3DPP_OK;
-}
-
-const char* fmm3dpp_last_error(const fmm3dpp_plan* plan) { return plan ? plan->err.c_str() : "null plan"; }
-
-int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* s) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan has no device");
-    if (plan->own_stream && plan->stream) {
-        (void)hipStreamSynchronize(plan->stream);
-        (void)hipStreamDestroy(plan->stream);
-    }
-    plan->stream = static_cast<hipStream_t>(s);
-    plan->own_stream = false;
-    return FMM3DPP_OK;
-}
-
-int fmm3dpp_precompute(fmm3dpp_plan* plan) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (plan->device >= 0) HIPCK(hipSetDevice(plan->device));
-    return ensure_ops(plan);
-}
-
-int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const double* y, const double* z) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (!x || !y || !z) FAIL(FMM3DPP_ERR_ARG, "null position array");
-    if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
-    plan->particles = false;
-    plan->host_struct_valid = false;
-    if (plan->device < 0) {
-        int rc = build_structure(plan->cfg, n, x, y, z, plan->st, plan->err);
-        if (rc) return rc;
-        plan->host_ms["tree_and_lists"] = plan->st.ms_tree + plan->st.ms_p2p + plan->st.ms_m2l;
-        plan->host_struct_valid = true;
-        plan->particles = true;
-        return FMM3DPP_OK;
-    }
-    HIPCK(hipSetDevice(plan->device));
-    hipStream_t s = plan->stream;
-    HIPCK(plan->d_inx.ensure(n * 8)); HIPCK(plan->d_iny.ensure(n * 8)); HIPCK(plan->d_inz.ensure(n * 8));
-    HIPCK(hipMemcpyAsync(plan->d_inx.p, x, n * 8, hipMemcpyHostToDevice, s));
-    HIPCK(hipMemcpyAsync(plan->d_iny.p, y, n * 8, hipMemcpyHostToDevice, s));
-    HIPCK(hipMemcpyAsync(plan->d_inz.p, z, n * 8, hipMemcpyHostToDevice, s));
-    int rc = build_device(plan, n, plan->d_inx.as<double>(), plan->d_iny.as<double>(), plan->d_inz.as<double>());
-    if (rc) return rc;
-    plan->particles = true;
-    return FMM3DPP_OK;
-}
-
-int fmm3dpp_set_particles_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y, const double* d_z) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
-    if (!d_x || !d_y || !d_z) FAIL(FMM3DPP_ERR_ARG, "null position array");
-    if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
-    plan->particles = false;
-    plan->host_struct_valid = false;
-    HIPCK(hipSetDevice(plan->device));
-    int rc = build_device(plan, n, d_x, d_y, d_z);
-    if (rc) return rc;
-    plan->particles = true;
-    return FMM3DPP_OK;
-}
-
-int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_phi, double* d_grad) {
-    if (!plan) return FMM3DPP_ERR_ARG;
-    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
-    if (!plan->particles) FAIL(FMM3DPP_ERR_STATE, "set_particles must precede evaluate");
-    if (!d_qin || !d_phi || !d_grad) FAIL(FMM3DPP_ERR_ARG, "null array");
-    HIPCK(hipSetDevice(plan->device));
-    int rc = ensure_ops(plan);
-    if (rc) return rc;
+// Far field of an evaluation (fmm.cpp:50-74 up to L2P): gather, P2M, M2M, M2L, L2L, L2P, on the
+// plan's stream.  Needs the tree (build_tree), not the near-field lists.
+int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
     hipStream_t s = plan->stream;
     const Operators& o = plan->ops;
     const GpuStructure& g = plan->gs;
@@ -365,6 +265,19 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
                           plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
                           plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
     HIPCK(hipEventRecord(plan->ev[7], s));
+    HIPCK(hipEventRecord(plan->ev[kStages + 1], s));
+    return FMM3DPP_OK;
+}
+
+// Near field and the final correction (fmm.cpp:75-80): P2P over the lists, dipole term, 1/(4 pi),
+// input order.  Needs build_lists.
+int eval_near(fmm3dpp_plan* plan, double* d_phi, double* d_grad) {
+    hipStream_t s = plan->stream;
+    const GpuStructure& g = plan->gs;
+    const int n = (int)g.n, nleaf = g.nleaf;
+    const CellsDev cells = plan->cells_dev();
+    const BodiesDev bod = plan->bodies_dev();
+    HIPCK(hipEventRecord(plan->ev[7], s));
     // P2P (fmm.cpp:259-294)
     if (!plan->mixed()) {
         HIPCK(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(),
@@ -388,6 +301,145 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     HIPCK(hipEventRecord(plan->ev[9], s));
     plan->ev_recorded = true;
     return FMM3DPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** out) {
+    if (!out) return FMM3DPP_ERR_ARG;
+    *out = nullptr;
+    std::string err;
+    int rc = validate_config(cfg, err);
+    if (rc) return rc;
+    std::unique_ptr<fmm3dpp_plan> plan(new fmm3dpp_plan());
+    plan->cfg = *cfg;
+    plan->device = device;
+    if (device != FMM3DPP_DEVICE_NONE) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FMM3DPP_ERR_DEVICE;
+        if (device < 0 || device >= count) return FMM3DPP_ERR_DEVICE;
+        if (hipSetDevice(device) != hipSuccess) return FMM3DPP_ERR_DEVICE;
+        if (hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
+        plan->own_stream = true;
+        for (int i = 0; i <= kStages + 1; i++)
+            if (hipEventCreate(&plan->ev[i]) != hipSuccess) return FMM3DPP_ERR_HIP;
+        if (hipStreamCreateWithFlags(&plan->list_stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
+    }
+    *out = plan.release();
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_plan_destroy(fmm3dpp_plan* plan) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device >= 0) {
+        (void)hipSetDevice(plan->device);
+        if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+        if (plan->list_stream) (void)hipStreamSynchronize(plan->list_stream);
+        for (int i = 0; i <= kStages + 1; i++) if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
+        if (plan->own_stream && plan->stream) (void)hipStreamDestroy(plan->stream);
+        if (plan->list_stream) (void)hipStreamDestroy(plan->list_stream);
+    }
+    delete plan;
+    return FMM3DPP_OK;
+}
+
+const char* fmm3dpp_last_error(const fmm3dpp_plan* plan) { return plan ? plan->err.c_str() : "null plan"; }
+
+int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* s) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan has no device");
+    if (plan->own_stream && plan->stream) {
+        (void)hipStreamSynchronize(plan->stream);
+        (void)hipStreamDestroy(plan->stream);
+    }
+    plan->stream = static_cast<hipStream_t>(s);
+    plan->own_stream = false;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_precompute(fmm3dpp_plan* plan) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device >= 0) HIPCK(hipSetDevice(plan->device));
+    return ensure_ops(plan);
+}
+
+int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const double* y, const double* z) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (!x || !y || !z) FAIL(FMM3DPP_ERR_ARG, "null position array");
+    if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
+    plan->particles = false;
+    plan->host_struct_valid = false;
+    if (plan->device < 0) {
+        int rc = build_structure(plan->cfg, n, x, y, z, plan->st, plan->err);
+        if (rc) return rc;
+        plan->host_ms["tree_and_lists"] = plan->st.ms_tree + plan->st.ms_p2p + plan->st.ms_m2l;
+        plan->host_struct_valid = true;
+        plan->particles = true;
+        return FMM3DPP_OK;
+    }
+    HIPCK(hipSetDevice(plan->device));
+    hipStream_t s = plan->stream;
+    HIPCK(plan->d_inx.ensure(n * 8)); HIPCK(plan->d_iny.ensure(n * 8)); HIPCK(plan->d_inz.ensure(n * 8));
+    HIPCK(hipMemcpyAsync(plan->d_inx.p, x, n * 8, hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(plan->d_iny.p, y, n * 8, hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(plan->d_inz.p, z, n * 8, hipMemcpyHostToDevice, s));
+    int rc = build_device(plan, n, plan->d_inx.as<double>(), plan->d_iny.as<double>(), plan->d_inz.as<double>());
+    if (rc) return rc;
+    plan->particles = true;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_set_particles_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y, const double* d_z) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
+    if (!d_x || !d_y || !d_z) FAIL(FMM3DPP_ERR_ARG, "null position array");
+    if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
+    plan->particles = false;
+    plan->host_struct_valid = false;
+    HIPCK(hipSetDevice(plan->device));
+    int rc = build_device(plan, n, d_x, d_y, d_z);
+    if (rc) return rc;
+    plan->particles = true;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_phi, double* d_grad) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
+    if (!plan->particles) FAIL(FMM3DPP_ERR_STATE, "set_particles must precede evaluate");
+    if (!d_qin || !d_phi || !d_grad) FAIL(FMM3DPP_ERR_ARG, "null array");
+    HIPCK(hipSetDevice(plan->device));
+    int rc = ensure_ops(plan);
+    if (rc) return rc;
+    rc = eval_far(plan, d_qin);
+    if (rc) return rc;
+    return eval_near(plan, d_phi, d_grad);
+}
+
+int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                         const double* d_q, double* d_phi, double* d_grad) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device < 0) FAIL(FMM3DPP_ERR_DEVICE, "plan was created without a device");
+    if (!d_x || !d_y || !d_z || !d_q || !d_phi || !d_grad) FAIL(FMM3DPP_ERR_ARG, "null array");
+    if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
+    plan->particles = false;
+    plan->host_struct_valid = false;
+    HIPCK(hipSetDevice(plan->device));
+    int rc = ensure_ops(plan);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    rc = build_device(plan, n, d_x, d_y, d_z, false);   // tree on the plan's stream, completed on return
+    if (rc) return rc;
+    plan->host_ms["tree"] = now_ms() - t0;
+    rc = eval_far(plan, d_q);                           // queued on the plan's stream
+    if (rc) return rc;
+    rc = plan->gs.build_lists(plan->cfg, plan->list_stream, plan->err);   // host waits on the list stream only
+    if (rc) return rc;
+    plan->host_ms.erase("tree_and_lists");   // the lists overlap the far field: no serial figure
+    plan->particles = true;
+    return eval_near(plan, d_phi, d_grad);
 }
 
 int fmm3dpp_evaluate(fmm3dpp_plan* plan, const double* q, double* phi, double* grad) {
@@ -548,6 +600,7 @@ double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
         return hipEventElapsedTime(&ms, p->ev[a], p->ev[b]) == hipSuccess ? (double)ms : -1.0;
     };
     if (s == "evaluate") return el(0, kStages);
+    if (s == "L2P") return el(6, kStages + 1);   // the near field may start later (fmm3dpp_solve_device)
     for (int i = 0; i < kStages; i++)
         if (s == kStageNames[i]) return el(i, i + 1);
     return -1.0;

@@ -23,6 +23,7 @@ DEVICE_NONE = -1
 EXPORTS = (
     "fmm3dpp_plan_create", "fmm3dpp_plan_destroy", "fmm3dpp_last_error", "fmm3dpp_precompute",
     "fmm3dpp_set_particles", "fmm3dpp_set_particles_device", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
+    "fmm3dpp_solve_device",
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
@@ -78,6 +79,7 @@ def lib():
         L.fmm3dpp_evaluate.argtypes = [vp, _dp, _dp, _dp]
         L.fmm3dpp_evaluate_device.argtypes = [vp, vp, vp, vp]
         L.fmm3dpp_set_stream.argtypes = [vp, vp]
+        L.fmm3dpp_solve_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
         L.fmm3dpp_get_counts.argtypes = [vp, ctypes.POINTER(Counts)]
         L.fmm3dpp_get_cells.argtypes = [vp, _ip, _dp]
         L.fmm3dpp_get_body_order.argtypes = [vp, _ip]
@@ -203,6 +205,13 @@ class LaplaceFMM:
         """Device pointers (ints), ordered on the plan's stream."""
         self._check(lib().fmm3dpp_evaluate_device(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(phi_ptr),
                                                    ctypes.c_void_p(grad_ptr)))
+
+    def solve_device(self, n, x_ptr, y_ptr, z_ptr, q_ptr, phi_ptr, grad_ptr):
+        """set_particles_device + evaluate_device in one call, the near-field list build overlapping the
+        far-field stages (fmm3dpp_solve_device); device pointers as ints."""
+        self.n = int(n)
+        self._check(lib().fmm3dpp_solve_device(self.h, self.n, *(ctypes.c_void_p(p) for p in
+                                                               (x_ptr, y_ptr, z_ptr, q_ptr, phi_ptr, grad_ptr))))
 
     def evaluate_adjoint(self, q, w, grad):
         """Backward of phi = evaluate(q) for w = dL/dphi and the forward gradient `grad`:

@@ -90,6 +90,14 @@ int fmm3dpp_evaluate(fmm3dpp_plan* plan, const double* q, double* phi, double* g
  * other than what the tree build needs).  d_grad is n x 3 row-major. */
 int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_q, double* d_phi, double* d_grad);
 
+/* One whole step on device arrays, LaplaceFMM::solve (src/fmm.cpp:13-81) without host copies:
+ * set_particles_device then evaluate_device, with the near-field list build (the frontier traversal
+ * and its CSR) running on a second stream of the plan while the far-field stages run on the plan's
+ * stream.  Results are those of the two calls.  Returns once the lists are built; the stages are
+ * ordered on the plan's stream as in evaluate_device. */
+int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y, const double* d_z,
+                         const double* d_q, double* d_phi, double* d_grad);
+
 /* Sets the HIP stream (hipStream_t) the plan launches on; NULL = the default stream. */
 int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* hip_stream);
 
@@ -204,7 +212,9 @@ double fmm3dpp_batch_timing_ms(const fmm3dpp_batch* batch, const char* name);
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
- * L2L, L2P, P2P, finalize, evaluate; host wall times: tree_and_lists (last set_particles),
+ * L2L, L2P, P2P, finalize, evaluate (for solve_device: from the first far-field launch, so it
+ * includes the list build that overlaps it); host wall times: tree_and_lists (last
+ * set_particles), tree (last solve_device, the tree build before the far field starts),
  * precompute.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);
 

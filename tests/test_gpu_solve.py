@@ -1,0 +1,47 @@
+"""fmm3dpp_solve_device (tree + lists + evaluation in one call, the near-field list build on a second
+stream beside the far field) against set_particles_device followed by evaluate_device: the same
+kernels on the same data, so the results are identical (the bound below is 0)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n, seed, clustered):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if clustered:
+        x = np.concatenate([rng.normal(0.1, 0.05, (n // 2, 3)), rng.random((n - n // 2, 3)) - 0.5])
+        x = (x + 0.5) % 1.0 - 0.5
+    else:
+        x = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    return x, q
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_device_equals_two_calls(precision):
+    import torch
+    import pyfmm3dpp
+    dev = torch.device("cuda", 0)
+    plan = pyfmm3dpp.LaplaceFMM(6, 48, 2, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_PHYSICAL, precision=precision, device=0)
+    ref = pyfmm3dpp.LaplaceFMM(6, 48, 2, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_PHYSICAL, precision=precision, device=0)
+    for n, seed, clustered in ((40000, 1, False), (25000, 2, True), (40000, 3, False)):   # reuse, shrink, regrow
+        print(f"case n={n} clustered={clustered} precision={precision}", flush=True)
+        x, q = _case(n, seed, clustered)
+        d = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(dev) for k in range(3)]
+        dq = torch.from_numpy(q).to(dev)
+        phi, grad = torch.empty(n, dtype=torch.float64, device=dev), torch.empty((n, 3), dtype=torch.float64, device=dev)
+        phi2, grad2 = torch.empty_like(phi), torch.empty_like(grad)
+        torch.cuda.synchronize()
+        plan.solve_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), dq.data_ptr(), phi.data_ptr(),
+                          grad.data_ptr())
+        ref.set_particles_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+        ref.evaluate_device(dq.data_ptr(), phi2.data_ptr(), grad2.data_ptr())
+        assert plan.timing_ms("evaluate") > 0 and ref.timing_ms("evaluate") > 0   # waits for both streams
+        torch.cuda.synchronize()
+        dp = float((phi - phi2).abs().max())
+        dg = float((grad - grad2).abs().max())
+        print(f"  max |diff| phi {dp:.3e} grad {dg:.3e}", flush=True)
+        assert dp == 0.0 and dg == 0.0
+        assert plan.counts() == ref.counts()
