@@ -410,7 +410,8 @@ def main():
     achieved = flops / (had_ms / 1e3) / 1e12
     mixed = args.path == "mixed"
     peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
-    roofline = {"kernel": "k_hadamard8_mfma<%s> (M2L 8x8 Hadamard on the matrix cores)" % ("float" if mixed else "double"),
+    roofline = {"kernel": "k_hadamard8_staged<%s> (M2L 8x8 Hadamard on the matrix cores, sources staged in LDS)"
+                          % ("float" if mixed else "double"),
                 "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": profiled_traffic(args.path),
                 "flops_per_launch": flops, "pairs": pairs,

@@ -50,6 +50,8 @@ struct Group {
     // merged device tables
     DBuf x, y, z, perm, cx, cy, cz, cr, depth, boff, bnum, leaf_cell, leaf_depth, iota, p2p_off, p2p_ent, m2m_in,
         m2m_out, l2l_in, l2l_out, img_a, img_b, child_slots, tar_scale, slot_src_T, img_src_T, img_tar, seg_off;
+    DBuf had_uniq, had_cnt, had_loc;   // Hadamard staging tables of the merged targets
+    HadTables had() const { return {had_uniq.as<int32_t>(), had_cnt.as<int32_t>(), had_loc.as<int32_t>()}; }
     DeviceOperators dop;
     M2LTiles up_tiles, down_tiles;   // transform tiles of the tree parents (0, 0: measured defaults)
     // work buffers
@@ -322,6 +324,14 @@ int build_group(fmm3dpp_batch* b, Group& g) {
     BHIP(upload(g.img_b, ib, s));
     BHIP(upload(g.img_tar, img_tar, s));
     BHIP(upload(g.seg_off, seg, s));
+    {   // Hadamard staging tables over the merged targets (tree parents only in mixed precision)
+        const int ntar_h = b->mixed() ? nl : ntar;
+        const size_t nch = (size_t)std::max(had_chunks(ntar_h), 1);
+        BHIP(g.had_uniq.ensure(nch * kHadUniqStride * 4)); BHIP(g.had_cnt.ensure(nch * 4));
+        BHIP(g.had_loc.ensure(nch * 26 * kHadChunk * 4));
+        BHIP(launch_hadamard_tables(ntar_h, ntar, g.slot_src_T.as<int32_t>(), g.had_uniq.as<int32_t>(),
+                                    g.had_cnt.as<int32_t>(), g.had_loc.as<int32_t>(), s));
+    }
     const int64_t iota_n = std::max<int64_t>(std::max<int64_t>(nleaf, 8 * (int64_t)g.max_level_cols), 1);
     BHIP(g.iota.ensure((size_t)iota_n * 4));
     k_iota32<<<(unsigned)std::min<int64_t>((iota_n + 255) / 256, 4096), 256, 0, s>>>(g.iota.as<int32_t>(), iota_n);
@@ -393,14 +403,16 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     const double* tw = g.dop.o_tw.as<double>();
     if (!mixed) {
         BHIP(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, g.Qk.p, s, g.up_tiles));
-        BHIP(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.dop.o_gk8.p, g.Qk.p, g.Pk.p, s));
+        BHIP(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), g.dop.o_gkA.p, g.Qk.p, g.Pk.p,
+                              s));
         BHIP(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                g.dop.o_gk27.as<double>(), g.Qk.as<double>(), g.Pk.as<double>(), s));
         BHIP(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s,
                              g.down_tiles));
     } else {
         BHIP(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, g.Qk.p, s, g.up_tiles));
-        BHIP(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.dop.o_gk8f.p, g.Qk.p, g.Pk.p, s));
+        BHIP(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), g.dop.o_gkA.p, g.Qk.p, g.Pk.p,
+                              s));
         BHIP(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, g.Pk.p, pck, s,
                              g.down_tiles));
         if (nimg > 0) {
@@ -437,12 +449,12 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     const double cyc = b->cfg[0].cycle;
     if (!mixed) {
         BHIP(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(), g.p2p_ent.as<int32_t>(),
-                        cyc, g.pot.as<double>(), g.gx.as<double>(), g.gy.as<double>(), g.gz.as<double>(), s));
+                        cyc, g.pot.as<double>(), g.gx.as<double>(), g.gy.as<double>(), g.gz.as<double>(), true, s));
     } else {
         BHIP(launch_pack_rel(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.rel.as<float4>(), s));
         BHIP(launch_p2p_f32(g.leaf_cell.as<int32_t>(), nleaf, cells, g.rel.as<float4>(), g.p2p_off.as<int64_t>(),
                             g.p2p_ent.as<int32_t>(), cyc, g.pot.as<double>(), g.gx.as<double>(), g.gy.as<double>(),
-                            g.gz.as<double>(), s));
+                            g.gz.as<double>(), true, s));
     }
     // dipole term per system, 1/(4 pi), input order
     const int nseg = (int)g.sys.size();

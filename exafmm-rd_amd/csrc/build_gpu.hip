@@ -769,6 +769,13 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     } else {
         CK(img_tar.ensure(4));
     }
+    // --- Hadamard staging tables over the targets of the 8x8 Hadamard
+    const int ntar_h = cfg.precision == FMM3DPP_PREC_MIXED ? nl : ntar;
+    const size_t nch = (size_t)std::max(had_chunks(ntar_h), 1);
+    CK(had_uniq.ensure(nch * kHadUniqStride * 4)); CK(had_cnt.ensure(nch * 4));
+    CK(had_loc.ensure(nch * 26 * kHadChunk * 4));
+    CK(launch_hadamard_tables(ntar_h, ntar, slot_src_T.as<int32_t>(), had_uniq.as<int32_t>(), had_cnt.as<int32_t>(),
+                              had_loc.as<int32_t>(), st));
     return FMM3DPP_OK;
 }
 

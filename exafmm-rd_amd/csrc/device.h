@@ -62,9 +62,26 @@ hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, cons
 struct M2LTiles { int sg = 0, fc = 0; };
 hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_child, const int32_t* inv_lat,
                          const double* twiddle, const double* qeq, void* Qk, hipStream_t st, M2LTiles tiles = {});
-// targets t < ntar, slot_src_T rows of stride ntab (-1: no source)
-hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
-                            const void* Qk, void* Pk, hipStream_t st);
+// Hadamard staging tables per chunk c of kHadChunk consecutive targets: the chunk's distinct sources
+// uniq[c * kHadUniqStride + r] (r < cnt[c], increasing) and each (pattern p, target c * kHadChunk +
+// tt)'s row among them, loc[(c * 26 + p) * kHadChunk + tt] (cnt[c] when it has no source).  A chunk
+// with more than kHadCap distinct sources is read from the spectra directly.
+constexpr int kHadChunk = 256, kHadCap = 767, kHadUniqStride = 26 * kHadChunk;
+inline int had_chunks(int ntar) { return (ntar + kHadChunk - 1) / kHadChunk; }
+struct HadTables {
+    const int32_t *uniq = nullptr, *cnt = nullptr, *loc = nullptr;
+};
+hipError_t launch_hadamard_tables(int ntar, int ntab, const int32_t* slot_src_T, int32_t* uniq, int32_t* cnt,
+                                  int32_t* loc, hipStream_t st);
+// the 8x8 kernel spectra gk8 ([26][F][64] complex) as matrix-core A operands in lane order, F x 26 x 64
+// vectors of 4 (float32 when f32, else float64)
+hipError_t launch_hadamard_lanes(bool f32, int F, const void* gk8, void* gkA, hipStream_t st);
+// targets t < ntar, slot_src_T rows of stride ntab (-1: no source), tables of launch_hadamard_tables
+// for the same (ntar, ntab, slot_src_T), gkA of launch_hadamard_lanes.  Chunks of at most cap
+// (<= kHadCap) distinct sources are staged through LDS, the others read the spectra directly; cap
+// -1 stages none.  The results do not depend on cap.
+hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, HadTables tab,
+                            const void* gkA, const void* Qk, void* Pk, hipStream_t st, int cap = kHadCap);
 // float64 1x27 image lattice; img_src_T rows have stride ntab, Qk/Pk hold the parents from index base on
 hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,
                              const double* gk27_sum, const double* Qk, double* Pk, hipStream_t st);
@@ -76,16 +93,18 @@ bool m2l_tiles_valid(int P, bool f32, bool down, M2LTiles tiles);
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
                            hipStream_t st);
-// list_ent[e] = source leaf cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle)
+// list_ent[e] = source leaf cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle).
+// accumulate: p += phi_near, g -= sum q d / r^3 (onto the far field); otherwise the near field alone
+// is stored (p = phi_near, g = -sum q d / r^3), for a finalize that adds the two.
 hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
                       const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
-                      hipStream_t st);
+                      bool accumulate, hipStream_t st);
 // float32 body records (x, y, z relative to the leaf centre, q) and the float32 P2P over them
 hipError_t launch_pack_rel(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, float4* rel,
                            hipStream_t st);
 hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, const float4* rel,
                           const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
-                          double* gy, double* gz, hipStream_t st);
+                          double* gy, double* gz, bool accumulate, hipStream_t st);
 // partial[5 * block + (0..4)] = sum q x, sum q y, sum q z, sum q |x|^2, sum q
 hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 5 * nblocks */, int nblocks, hipStream_t st);
 // dL/dq and dL/dx from w = dL/dphi, grad phi, and (psi, grad psi) = the evaluation with charges w
@@ -93,10 +112,15 @@ hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const
                                   double c, double out_scale, const double* qin, const double* win,
                                   const double* grad_phi, const double* psi, const double* grad_psi, double* dq,
                                   double* dx, hipStream_t st);
-// mode: FMM3DPP_DIPOLE_* (0 off, 1 reference, 2 infinite-lattice gauge)
+// Near-field results stored apart from the far field (P2P with accumulate = false); all null: none.
+struct NearDev {
+    const double *p = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr;
+};
+// mode: FMM3DPP_DIPOLE_* (0 off, 1 reference, 2 infinite-lattice gauge).  With `near` set, phi =
+// p + near.p and grad = g + near.g, the same roundings as the P2P accumulating onto p and g.
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
-                           const double* gy, const double* gz, const double* partial, int nblocks, int mode,
-                           double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
+                           const double* gy, const double* gz, NearDev near, const double* partial, int nblocks,
+                           int mode, double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
 
 // Batches: moments and the final correction per segment (system) of the tree-ordered bodies;
 // partial holds nseg x nblocks x 5 doubles, moments nseg x 5.  A fill kernel instead of

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "dbuf.h"
+#include "device.h"
 #include "structure.h"
 
 namespace fmm3dpp {
@@ -14,6 +15,7 @@ struct DeviceOperators {
     DBuf o_m2m_all, o_l2l;           // the 8 octant operators, concatenated
     DBuf o_m2m_img, o_l2l_img;       // image-level M2M (sum over the 27 children) and L2L; zeros without images
     DBuf o_gk8, o_gk8f;              // 8x8 M2L kernel spectra, float64 and (mixed precision) float32
+    DBuf o_gkA;                      // the same as matrix-core A operands in lane order, in the Hadamard's type
     DBuf o_gk27;                     // 1x27 image-lattice kernel spectra, summed over the 27 children
     DBuf o_usurf;                    // unit check/equivalent surface
     DBuf o_invlat;                   // P^3 grid corner (k * P + j) * P + i -> surface point, -1 inside
@@ -45,6 +47,8 @@ struct DeviceOperators {
         up(o_usurf, o.unit_surf);
         up(o_invlat, inv);
         up(o_tw, o.twiddle);
+        if (e == hipSuccess) e = o_gkA.ensure((size_t)o.F * 26 * 64 * 4 * (mixed ? sizeof(float) : sizeof(double)));
+        if (e == hipSuccess) e = launch_hadamard_lanes(mixed, o.F, mixed ? o_gk8f.p : o_gk8.p, o_gkA.p, s);
         return e;
     }
 };

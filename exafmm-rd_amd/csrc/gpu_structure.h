@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "dbuf.h"
+#include "device.h"
 #include "fmm3dpp.h"
 #include "structure.h"
 
@@ -36,6 +37,10 @@ struct GpuStructure {
     DBuf m2m_in, m2m_out, l2l_in, l2l_out, img_a, img_b;
     // M2L Hadamard tables (sources = targets = non-leaf cells, then image cells)
     DBuf nl_idx, nl_cells, child_slots, tar_scale, slot_src_T, img_src_T, img_tar, img_scale;
+    // staging tables of the 8x8 Hadamard over its targets (the non-leaf cells in mixed precision, all
+    // M2L targets in float64): distinct sources and rows per chunk of targets
+    DBuf had_uniq, had_cnt, had_loc;
+    HadTables had() const { return {had_uniq.as<int32_t>(), had_cnt.as<int32_t>(), had_loc.as<int32_t>()}; }
     // P2P rows (target, source, offset index) and per-leaf CSR (source << 5 | offset index)
     DBuf rows_tc, rows_sc, rows_oi, p2p_off, p2p_ent;
     size_t rows_cap = 0;

@@ -159,13 +159,20 @@ __device__ void sum_moments(const double* __restrict__ partial, int nblocks, dou
 __global__ __launch_bounds__(256) void k_finalize(int n, const int32_t* __restrict__ perm, BodiesDev b,
                                                   const double* __restrict__ p, const double* __restrict__ gx,
                                                   const double* __restrict__ gy, const double* __restrict__ gz,
-                                                  const double* __restrict__ partial, int nblocks, int mode, double cycle,
-                                                  double out_scale, double* __restrict__ phi, double* __restrict__ grad) {
+                                                  NearDev near, const double* __restrict__ partial, int nblocks,
+                                                  int mode, double cycle, double out_scale, double* __restrict__ phi,
+                                                  double* __restrict__ grad) {
     __shared__ double M[kMoments];
     sum_moments(partial, nblocks, M);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double pv = p[i], g0 = gx[i], g1 = gy[i], g2 = gz[i];
+    if (near.p) {   // far + near, the sums the accumulating P2P forms
+        pv += near.p[i];
+        g0 += near.gx[i];
+        g1 += near.gy[i];
+        g2 += near.gz[i];
+    }
     if (mode != 0) {
         const double coef = 4 * M_PI / (3 * cycle * cycle * cycle);
         if (mode == 1) {
@@ -322,12 +329,13 @@ hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial, int nblock
 }
 
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
-                           const double* gy, const double* gz, const double* partial, int nblocks, int mode,
-                           double cycle, double out_scale, double* phi, double* grad, hipStream_t st) {
+                           const double* gy, const double* gz, NearDev near, const double* partial, int nblocks,
+                           int mode, double cycle, double out_scale, double* phi, double* grad, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     if (!perm || !p || !gx || !gy || !gz || !partial || !phi || !grad || !valid(b)) return hipErrorInvalidValue;
-    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, partial, nblocks, mode, cycle, out_scale,
-                                                phi, grad);
+    if (near.p && (!near.gx || !near.gy || !near.gz)) return hipErrorInvalidValue;
+    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, near, partial, nblocks, mode, cycle,
+                                                out_scale, phi, grad);
     return hipGetLastError();
 }
 

@@ -132,36 +132,19 @@ __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Hadamard 8x8 (src/kernel.cpp:1012-1082, include/c2c.h:201-296) on the matrix cores.  Targets
-// t < ntar, source table rows slot_src_T[p * ntab + t] (-1: no source; sources < ntab).  For one
-// frequency f:  Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
-// Each complex 8x8 G_p acts as the real 16x16 matrix M on interleaved (re, im) vectors:
-//   M[2it][2is] = M[2it+1][2is+1] = Re G[is][it],  M[2it][2is+1] = -Im G,  M[2it+1][2is] = Im G.
-// A wave multiplies M (A operand, rows = output reals) by 16 targets' source vectors (B operand,
-// columns = targets) with four K=4 steps; lane (n, kr) feeds reals 4kr..4kr+3 of target n's source
-// vector, so the K order is permuted (step kk, lane group kr -> real 4kr + kk) identically in A and
-// B.  The A rows are permuted too, so that output register i of lane (n, kr) is real 4kr + i of
-// target n whatever the accumulator layout of the type: A row d_row(g, i) holds output real 4g + i.
-// The workgroup first writes each lane's four A values of all 26 patterns to LDS (one 4-vector read
-// per pattern afterwards).  The 26 patterns are software-pipelined: source indices are read two
-// patterns ahead and source vectors one pattern ahead, from row 0 when there is no source (the
-// vector is zeroed before use), so no load waits on a branch.  TT tiles of 16 targets per wave,
-// 4 waves per workgroup, one frequency per workgroup; blocks are remapped so that consecutive
-// frequency slices run on one XCD.
-template <typename T, int TT>
-__global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int ntab, int nchunk, int total,
-                                                        const int32_t* __restrict__ slot_src_T,
-                                                        const T* __restrict__ gk8, const T* __restrict__ Qk,
-                                                        T* __restrict__ Pk) {
-    __shared__ vec4<T> as[26 * 64];
-    const int per = (total + 7) / 8;
-    const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
-    if (Lb >= total) return;                                    // uniform over the workgroup
-    const int f = Lb / nchunk, chunk = Lb % nchunk;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < 26 * 64; e += 256) {                 // A operands of (pattern, lane)
+// A operand of lane l for pattern p at frequency f, laid out as the Hadamard reads it:
+// gkA[(f * 26 + p) * 64 + l].  A complex 8x8 block G acts on interleaved (re, im) vectors as the real
+// 16x16 matrix M[2it][2is] = M[2it+1][2is+1] = Re G[is][it], M[2it][2is+1] = -Im G, M[2it+1][2is] =
+// Im G.  Lane (nl, kl) holds A row nl, K columns 4kl..4kl+3 in the permuted K order of the B operand
+// (step k, lane group kl -> real 4kl + k); the rows are permuted so that output register i of lane
+// (n, kr) is real 4kr + i of target n whatever the accumulator layout of the type (A row
+// d_row(g, i) holds output real 4g + i).
+template <typename T>
+__global__ __launch_bounds__(256) void k_hadamard_lanes(int F, const T* __restrict__ gk8, vec4<T>* __restrict__ gkA) {
+    const int f = blockIdx.x;
+    for (int e = threadIdx.x; e < 26 * 64; e += 256) {
         const int p = e >> 6, l = e & 63, nl = l & 15, kl = l >> 4;
-        int o = nl;                                             // output real o = 2 it + (re 0 / im 1) of A row nl
+        int o = nl;   // output real o = 2 it + (re 0 / im 1) of A row nl
         for (int g = 0; g < 4; g++)
             for (int i = 0; i < 4; i++)
                 if (d_row<T>(g, i) == nl) o = 4 * g + i;
@@ -169,58 +152,236 @@ __global__ __launch_bounds__(256) void k_hadamard8_mfma(int F, int ntar, int nta
         const T* gp = gk8 + ((size_t)p * F + f) * 128;
         const T g0r = gp[((2 * kl) * 8 + it) * 2], g0i = gp[((2 * kl) * 8 + it) * 2 + 1];
         const T g1r = gp[((2 * kl + 1) * 8 + it) * 2], g1i = gp[((2 * kl + 1) * 8 + it) * 2 + 1];
-        as[e] = im ? vec4<T>{g0i, g0r, g1i, g1r} : vec4<T>{g0r, -g0i, g1r, -g1i};
+        gkA[(size_t)f * 26 * 64 + e] = im ? vec4<T>{g0i, g0r, g1i, g1r} : vec4<T>{g0r, -g0i, g1r, -g1i};
+    }
+}
+
+// Staging tables of the Hadamard, one workgroup per chunk c of kHadChunk consecutive targets: the
+// distinct sources of all 26 patterns of the chunk in increasing order (uniq[c * kHadUniqStride + r],
+// r < cnt[c]) and, per (pattern, target), the row of its source among them
+// (loc[(c * 26 + p) * kHadChunk + tt]; cnt[c] when there is none).  Bitonic sort of the chunk's
+// 26 x kHadChunk source indices in LDS, unique marking, a block scan for the ranks, then a binary
+// search per entry.
+constexpr int kHadKeys = 8192;   // 26 * kHadChunk rounded up to a power of two
+static_assert(26 * kHadChunk <= kHadKeys, "sort size");
+__global__ __launch_bounds__(1024) void k_hadamard_tables(int ntar, int ntab, const int32_t* __restrict__ slot_src_T,
+                                                          int32_t* __restrict__ uniq, int32_t* __restrict__ cnt,
+                                                          int32_t* __restrict__ loc) {
+    __shared__ int32_t keys[kHadKeys];
+    __shared__ int32_t uq[26 * kHadChunk];
+    __shared__ int32_t part[1024];
+    constexpr int32_t kNone = 0x7fffffff;
+    const int c = blockIdx.x, tid = threadIdx.x;
+    auto source = [&](int i) {   // entry i = p * kHadChunk + tt of the chunk, -1 when none
+        const int p = i / kHadChunk, t = c * kHadChunk + i % kHadChunk;
+        return t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
+    };
+    for (int i = tid; i < kHadKeys; i += 1024) {
+        const int s = i < 26 * kHadChunk ? source(i) : -1;
+        keys[i] = s < 0 ? kNone : s;
+    }
+    __syncthreads();
+    for (int k = 2; k <= kHadKeys; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < kHadKeys; i += 1024) {
+                const int ij = i ^ j;
+                if (ij > i) {
+                    const int32_t a = keys[i], b = keys[ij];
+                    if ((a > b) == ((i & k) == 0)) {
+                        keys[i] = b;
+                        keys[ij] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    constexpr int PER = kHadKeys / 1024;   // consecutive keys per thread
+    int mine = 0;
+    for (int e = 0; e < PER; e++) {
+        const int i = tid * PER + e;
+        mine += keys[i] != kNone && (i == 0 || keys[i] != keys[i - 1]);
+    }
+    part[tid] = mine;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan of the per-thread counts
+        const int v = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    const int total = part[1023];
+    int r = part[tid] - mine;
+    int32_t* un = uniq + (size_t)c * kHadUniqStride;
+    for (int e = 0; e < PER; e++) {
+        const int i = tid * PER + e;
+        if (keys[i] != kNone && (i == 0 || keys[i] != keys[i - 1])) {
+            uq[r] = keys[i];
+            un[r] = keys[i];
+            r++;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 26 * kHadChunk; i += 1024) {
+        const int s = source(i);
+        int l = total;
+        if (s >= 0) {   // s is among uq[0, total)
+            int lo = 0, hi = total - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (uq[mid] < s) lo = mid + 1;
+                else hi = mid;
+            }
+            l = lo;
+        }
+        loc[(size_t)c * 26 * kHadChunk + i] = l;
+    }
+    if (tid == 0) cnt[c] = total;
+}
+
+// Hadamard 8x8 (src/kernel.cpp:1012-1082, include/c2c.h:201-296) on the matrix cores.  Targets
+// t < ntar, source table rows slot_src_T[p * ntab + t] (-1: no source; sources < ntab).  For one
+// frequency f:  Pk[t][it] = sum over the 26 neighbour patterns p of sum_is G_p[is][it] Qk[src(t,p)][is].
+// One workgroup per (frequency, chunk of kHadChunk targets): 8 waves of 2 tiles of 16 targets.  A
+// wave multiplies the real 16x16 form of G_p (A operand, lane order of k_hadamard_lanes, staged in
+// LDS per frequency) by 16 targets' source vectors (B operand, columns = targets) with four K=4
+// steps; lane (n, kr) feeds reals 4kr..4kr+3 of target n's source vector.  The chunk's distinct
+// source vectors at f (at most cap <= kHadCap, k_hadamard_tables) are first copied into LDS with one
+// zero row for the missing sources, so the 26 x 256 source reads of the chunk come from LDS; a chunk
+// with more distinct sources reads them from Qk instead (missing sources read row 0 and are zeroed in
+// registers).  Both forms multiply the same values in the same order.  Source rows and table entries
+// are read ahead of the matrix ops of the pattern before.  Blocks are remapped so that consecutive
+// frequency slices run on one XCD.
+template <typename T>
+__global__ __launch_bounds__(512) void k_hadamard8_staged(int F, int ntar, int ntab, int nchunk, int total,
+                                                          const int32_t* __restrict__ slot_src_T,
+                                                          const int32_t* __restrict__ uniq,
+                                                          const int32_t* __restrict__ cnt_tab,
+                                                          const int32_t* __restrict__ loc,
+                                                          const vec4<T>* __restrict__ gkA, const T* __restrict__ Qk,
+                                                          T* __restrict__ Pk, int cap) {
+    constexpr int TT = 2, NT = 512, CT = kHadChunk;
+    static_assert(16 * TT * (NT / 64) == CT, "tiles cover the chunk");
+    constexpr int MAXU = (kHadCap + 1) * 4 / NT;   // staged vec4 per thread
+    static_assert(MAXU * NT == (kHadCap + 1) * 4, "staging covers the rows");
+    extern __shared__ __align__(16) char smem[];
+    vec4<T>* as = reinterpret_cast<vec4<T>*>(smem);   // 26 x 64 A operands
+    vec4<T>* bs = as + 26 * 64;                        // staged source rows, 4 vec4 each
+    const int per = (total + 7) / 8;
+    const int Lb = (blockIdx.x % 8) * per + blockIdx.x / 8;   // consecutive logical blocks on one XCD
+    if (Lb >= total) return;                                    // uniform over the workgroup
+    const int f = Lb / nchunk, c = Lb % nchunk;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < 26 * 64; e += NT) as[e] = gkA[(size_t)f * 26 * 64 + e];
+    // 32-bit element offsets: the launcher checks ntab * F * 4 < 2^32
+    const vec4<T>* q4 = reinterpret_cast<const vec4<T>*>(Qk) + (size_t)f * 4;
+    const uint32_t rowstride = (uint32_t)F * 4u;
+    const int cnt = cnt_tab[c];
+    const bool staged = cnt <= cap;   // uniform over the workgroup; cap <= kHadCap
+    if (staged) {
+        const int32_t* un = uniq + (size_t)c * kHadUniqStride;
+        int src[MAXU];
+        vec4<T> v[MAXU];
+#pragma unroll
+        for (int u = 0; u < MAXU; u++) {
+            const int i = tid + NT * u;
+            src[u] = (i >> 2) < cnt ? un[i >> 2] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < MAXU; u++) {
+            const int i = tid + NT * u;
+            v[u] = src[u] >= 0 ? q4[(uint32_t)src[u] * rowstride + (i & 3)] : vec4<T>{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < MAXU; u++) {
+            const int i = tid + NT * u;
+            if ((i >> 2) <= cnt) bs[i] = v[u];   // rows 0..cnt, row cnt zero
+        }
     }
     __syncthreads();
     const int n = lane & 15, kr = lane >> 4;
-    const int tbase = (chunk * 4 + wave) * 16 * TT;
-    // 32-bit element offsets: the launcher checks ntab * F * 4 < 2^32
-    const vec4<T>* q4 = reinterpret_cast<const vec4<T>*>(Qk) + (size_t)f * 4 + kr;   // + src * F * 4
-    const uint32_t rowstride = (uint32_t)F * 4u;
-    int tcl[TT];
-#pragma unroll
-    for (int j = 0; j < TT; j++) tcl[j] = min(tbase + 16 * j + n, ntar - 1);   // lanes past ntar store nothing
-    auto load_idx = [&](int p, int* sidx) {
-#pragma unroll
-        for (int j = 0; j < TT; j++) sidx[j] = slot_src_T[(size_t)p * ntab + tcl[j]];
-    };
-    auto load_vec = [&](const int* sidx, vec4<T>* b) {
-#pragma unroll
-        for (int j = 0; j < TT; j++) b[j] = q4[(uint32_t)max(sidx[j], 0) * rowstride];
-    };
-    vec4<T> acc[TT], bv[TT], bn[TT];
-    int s0[TT], s1[TT], s2[TT];
+    const int tw = wave * 16 * TT;   // this wave's first target in the chunk
+    vec4<T> acc[TT];
 #pragma unroll
     for (int j = 0; j < TT; j++) acc[j] = vec4<T>{0, 0, 0, 0};
-    load_idx(0, s0);
-    load_vec(s0, bv);
-    load_idx(1, s1);
-#pragma unroll   // the buffer rotation below becomes register renaming
-    for (int p = 0; p < 26; p++) {
-        // keep the stages in place: the scheduler would otherwise hoist the next pattern's address
-        // arithmetic, and with it a wait on the loads just issued, into this one
-        __builtin_amdgcn_sched_barrier(0);
-        if (p + 1 < 26) load_vec(s1, bn);
-        if (p + 2 < 26) load_idx(p + 2, s2);
-        __builtin_amdgcn_sched_barrier(0);   // issue those loads before this pattern's matrix ops
-        const vec4<T> a = as[p * 64 + lane];
-#pragma unroll
-        for (int j = 0; j < TT; j++)
-            if (s0[j] < 0) bv[j] = vec4<T>{0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-#pragma unroll
-            for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a[k], bv[j][k], acc[j]);
+    if (staged) {
+        const int32_t* lc = loc + (size_t)c * 26 * CT + tw + n;
+        int l0[TT], l1[TT], l2[TT];
 #pragma unroll
         for (int j = 0; j < TT; j++) {
-            bv[j] = bn[j];
-            s0[j] = s1[j];
-            s1[j] = s2[j];
+            l0[j] = lc[16 * j];
+            l1[j] = lc[CT + 16 * j];
+        }
+        vec4<T> a0 = as[lane], a1;
+        vec4<T> b0[TT], b1[TT];
+#pragma unroll
+        for (int j = 0; j < TT; j++) b0[j] = bs[l0[j] * 4 + kr];
+#pragma unroll   // the buffer rotation below becomes register renaming
+        for (int p = 0; p < 26; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (p + 2 < 26) {
+#pragma unroll
+                for (int j = 0; j < TT; j++) l2[j] = lc[(p + 2) * CT + 16 * j];
+            }
+            if (p + 1 < 26) {
+                a1 = as[(p + 1) * 64 + lane];
+#pragma unroll
+                for (int j = 0; j < TT; j++) b1[j] = bs[l1[j] * 4 + kr];
+            }
+            __builtin_amdgcn_sched_barrier(0);   // issue those reads before this pattern's matrix ops
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a0[k], b0[j][k], acc[j]);
+            a0 = a1;
+#pragma unroll
+            for (int j = 0; j < TT; j++) {
+                b0[j] = b1[j];
+                l1[j] = l2[j];
+            }
+        }
+    } else {
+        const vec4<T>* qk = q4 + kr;
+        int tcl[TT];
+#pragma unroll
+        for (int j = 0; j < TT; j++) tcl[j] = min(c * CT + tw + 16 * j + n, ntar - 1);   // lanes past ntar store nothing
+        auto load_idx = [&](int p, int* sidx) {
+#pragma unroll
+            for (int j = 0; j < TT; j++) sidx[j] = slot_src_T[(size_t)p * ntab + tcl[j]];
+        };
+        auto load_vec = [&](const int* sidx, vec4<T>* b) {
+#pragma unroll
+            for (int j = 0; j < TT; j++) b[j] = qk[(uint32_t)max(sidx[j], 0) * rowstride];
+        };
+        vec4<T> bv[TT], bn[TT];
+        int s0[TT], s1[TT], s2[TT];
+        load_idx(0, s0);
+        load_vec(s0, bv);
+        load_idx(1, s1);
+#pragma unroll
+        for (int p = 0; p < 26; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (p + 1 < 26) load_vec(s1, bn);
+            if (p + 2 < 26) load_idx(p + 2, s2);
+            __builtin_amdgcn_sched_barrier(0);
+            const vec4<T> a = as[p * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < TT; j++)
+                if (s0[j] < 0) bv[j] = vec4<T>{0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < TT; j++) acc[j] = mfma16x16x4(a[k], bv[j][k], acc[j]);
+#pragma unroll
+            for (int j = 0; j < TT; j++) {
+                bv[j] = bn[j];
+                s0[j] = s1[j];
+                s1[j] = s2[j];
+            }
         }
     }
 #pragma unroll
     for (int j = 0; j < TT; j++) {
-        const int t = tbase + 16 * j + n;
+        const int t = c * CT + tw + 16 * j + n;
         if (t < ntar) *reinterpret_cast<vec4<T>*>(Pk + ((size_t)t * F + f) * 16 + 4 * kr) = acc[j];
     }
 }
@@ -443,16 +604,23 @@ hipError_t down_p(int S, int ntar, const int32_t* tar_child, const double* tar_s
 }
 
 template <typename T>
-hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, const void* gk8,
-                       const void* Qk, void* Pk, hipStream_t st) {
-    constexpr int TT = 4;
+size_t hadamard8_lds() { return (size_t)(26 * 64 + (kHadCap + 1) * 4) * sizeof(vec4<T>); }
+
+template <typename T>
+hipError_t hadamard8_t(int F, int ntar, int ntab, const int32_t* slot_src_T, HadTables tab, const void* gkA,
+                       const void* Qk, void* Pk, int cap, hipStream_t st) {
+    static std::atomic<uint32_t> attr{0};
     if ((uint64_t)ntab * (uint64_t)F * 4u >= (uint64_t(1) << 32)) return hipErrorInvalidValue;   // 32-bit offsets
-    const int nchunk = (ntar + 64 * TT - 1) / (64 * TT);
+    hipError_t e = allow_large_lds((const void*)k_hadamard8_staged<T>, attr);
+    if (e != hipSuccess) return e;
+    const int nchunk = had_chunks(ntar);
     const int total = nchunk * F;
     const int per = (total + 7) / 8;
-    k_hadamard8_mfma<T, TT><<<per * 8, 256, 0, st>>>(F, ntar, ntab, nchunk, total, slot_src_T,
-                                                     static_cast<const T*>(gk8), static_cast<const T*>(Qk),
-                                                     static_cast<T*>(Pk));
+    k_hadamard8_staged<T><<<per * 8, 512, hadamard8_lds<T>(), st>>>(F, ntar, ntab, nchunk, total, slot_src_T, tab.uniq,
+                                                                     tab.cnt, tab.loc,
+                                                                     static_cast<const vec4<T>*>(gkA),
+                                                                     static_cast<const T*>(Qk), static_cast<T*>(Pk),
+                                                                     std::min(cap, kHadCap));
     return hipGetLastError();
 }
 
@@ -483,12 +651,30 @@ hipError_t launch_m2l_up(int P, bool f32, int S, int nsrc, const int32_t* src_ch
     }
 }
 
-hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T,
-                            const void* gk8, const void* Qk, void* Pk, hipStream_t st) {
+hipError_t launch_hadamard8(bool f32, int F, int ntar, int ntab, const int32_t* slot_src_T, HadTables tab,
+                            const void* gkA, const void* Qk, void* Pk, hipStream_t st, int cap) {
     if (ntar <= 0) return hipSuccess;
-    if (!slot_src_T || !gk8 || !Qk || !Pk || ntab < ntar) return hipErrorInvalidValue;
-    return f32 ? hadamard8_t<float>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st)
-               : hadamard8_t<double>(F, ntar, ntab, slot_src_T, gk8, Qk, Pk, st);
+    if (!slot_src_T || !tab.uniq || !tab.cnt || !tab.loc || !gkA || !Qk || !Pk || ntab < ntar)
+        return hipErrorInvalidValue;
+    return f32 ? hadamard8_t<float>(F, ntar, ntab, slot_src_T, tab, gkA, Qk, Pk, cap, st)
+               : hadamard8_t<double>(F, ntar, ntab, slot_src_T, tab, gkA, Qk, Pk, cap, st);
+}
+
+hipError_t launch_hadamard_tables(int ntar, int ntab, const int32_t* slot_src_T, int32_t* uniq, int32_t* cnt,
+                                  int32_t* loc, hipStream_t st) {
+    if (ntar <= 0) return hipSuccess;
+    if (!slot_src_T || !uniq || !cnt || !loc || ntab < ntar) return hipErrorInvalidValue;
+    k_hadamard_tables<<<had_chunks(ntar), 1024, 0, st>>>(ntar, ntab, slot_src_T, uniq, cnt, loc);
+    return hipGetLastError();
+}
+
+hipError_t launch_hadamard_lanes(bool f32, int F, const void* gk8, void* gkA, hipStream_t st) {
+    if (F <= 0 || !gk8 || !gkA) return hipErrorInvalidValue;
+    if (f32)
+        k_hadamard_lanes<float><<<F, 256, 0, st>>>(F, static_cast<const float*>(gk8), static_cast<f32x4*>(gkA));
+    else
+        k_hadamard_lanes<double><<<F, 256, 0, st>>>(F, static_cast<const double*>(gk8), static_cast<f64x4*>(gkA));
+    return hipGetLastError();
 }
 
 hipError_t launch_hadamard27(int F, int nimg, const int32_t* img_tar, const int32_t* img_src_T, int ntab, int base,

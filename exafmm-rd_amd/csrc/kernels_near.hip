@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64) void k_p2p_f64(const int32_t* __restrict__ leaf
                                                 const int64_t* __restrict__ list_off,
                                                 const int32_t* __restrict__ list_ent, double cycle,
                                                 double* __restrict__ p, double* __restrict__ gx,
-                                                double* __restrict__ gy, double* __restrict__ gz) {
+                                                double* __restrict__ gy, double* __restrict__ gz, bool accumulate) {
     __shared__ int m_sb[64], m_ns[64], m_oi[64];
     __shared__ double4 sbuf[64];
     const int li = blockIdx.x, lane = threadIdx.x;
@@ -113,10 +113,17 @@ __global__ __launch_bounds__(64) void k_p2p_f64(const int32_t* __restrict__ leaf
         ay = fold_groups(ay, wb);
         az = fold_groups(az, wb);
         if (active && g == 0) {
-            p[bi] += ap;
-            gx[bi] -= ax;
-            gy[bi] -= ay;
-            gz[bi] -= az;
+            if (accumulate) {
+                p[bi] += ap;
+                gx[bi] -= ax;
+                gy[bi] -= ay;
+                gz[bi] -= az;
+            } else {
+                p[bi] = ap;
+                gx[bi] = -ax;
+                gy[bi] = -ay;
+                gz[bi] = -az;
+            }
         }
     }
 }
@@ -143,7 +150,7 @@ __global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf
                                                 const float4* __restrict__ rel, const int64_t* __restrict__ list_off,
                                                 const int32_t* __restrict__ list_ent, double cycle,
                                                 double* __restrict__ p, double* __restrict__ gx,
-                                                double* __restrict__ gy, double* __restrict__ gz) {
+                                                double* __restrict__ gy, double* __restrict__ gz, bool accumulate) {
     __shared__ int m_sb[64], m_ns[64];
     __shared__ float m_dx[64], m_dy[64], m_dz[64];
     __shared__ float4 sbuf[64];
@@ -213,10 +220,17 @@ __global__ __launch_bounds__(64) void k_p2p_f32(const int32_t* __restrict__ leaf
         ay = fold_groups(ay, wb);
         az = fold_groups(az, wb);
         if (active && g == 0) {
-            p[bi] += ap;
-            gx[bi] -= ax;
-            gy[bi] -= ay;
-            gz[bi] -= az;
+            if (accumulate) {
+                p[bi] += ap;
+                gx[bi] -= ax;
+                gy[bi] -= ay;
+                gz[bi] -= az;
+            } else {
+                p[bi] = ap;
+                gx[bi] = -ax;
+                gy[bi] = -ay;
+                gz[bi] = -az;
+            }
         }
     }
 }
@@ -366,11 +380,11 @@ hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell,
 
 hipError_t launch_p2p(const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b, const int64_t* list_off,
                       const int32_t* list_ent, double cycle, double* p, double* gx, double* gy, double* gz,
-                      hipStream_t st) {
+                      bool accumulate, hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
     if (!leaf_cell || !list_off || !list_ent || !p || !gx || !gy || !gz || !valid(cells) || !valid(b))
         return hipErrorInvalidValue;
-    k_p2p_f64<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_ent, cycle, p, gx, gy, gz);
+    k_p2p_f64<<<nleaf, 64, 0, st>>>(leaf_cell, cells, b, list_off, list_ent, cycle, p, gx, gy, gz, accumulate);
     return hipGetLastError();
 }
 
@@ -384,11 +398,11 @@ hipError_t launch_pack_rel(const int32_t* leaf_cell, int nleaf, CellsDev cells, 
 
 hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, const float4* rel,
                           const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
-                          double* gy, double* gz, hipStream_t st) {
+                          double* gy, double* gz, bool accumulate, hipStream_t st) {
     if (nleaf <= 0) return hipSuccess;
     if (!leaf_cell || !rel || !list_off || !list_ent || !p || !gx || !gy || !gz || !valid(cells))
         return hipErrorInvalidValue;
-    k_p2p_f32<<<nleaf, 64, 0, st>>>(leaf_cell, cells, rel, list_off, list_ent, cycle, p, gx, gy, gz);
+    k_p2p_f32<<<nleaf, 64, 0, st>>>(leaf_cell, cells, rel, list_off, list_ent, cycle, p, gx, gy, gz, accumulate);
     return hipGetLastError();
 }
 

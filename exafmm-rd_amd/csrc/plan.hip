@@ -27,6 +27,9 @@ using namespace fmm3dpp;
 namespace {
 const char* kStageNames[] = {"P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize"};
 constexpr int kStages = 9;
+// Events: 0..kStages are the stage boundaries on the plan's stream (7: end of L2P, 8: near field
+// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream; kEvFork: charges in tree order.
+constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvFork = kStages + 3, kEvents = kStages + 4;
 }  // namespace
 
 struct fmm3dpp_plan {
@@ -40,11 +43,13 @@ struct fmm3dpp_plan {
     Structure st;              // host tree/lists (structure-only plans; export cache for device plans)
     GpuStructure gs;           // device tree/lists (device plans)
     std::map<std::string, double> host_ms;
-    hipEvent_t ev[kStages + 2] = {};   // stage boundaries; ev[kStages + 1]: end of the far field
-    hipStream_t list_stream = nullptr;   // near-field list build beside the far field (fmm3dpp_solve_device)
+    hipEvent_t ev[kEvents] = {};
+    hipStream_t side_stream = nullptr;   // near field (lists, P2P) beside the far field
     bool ev_recorded = false;
+    int had_cap = kHadCap;   // distinct sources a Hadamard chunk may stage through LDS (fmm3dpp_set_hadamard_staging)
 
     DBuf d_inx, d_iny, d_inz, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
+    DBuf d_pn, d_gxn, d_gyn, d_gzn;   // near field, written by the P2P on the side stream
     DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
     DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
     DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
@@ -129,6 +134,8 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
     const size_t nc = (size_t)g.ncells;
     HIPCK(plan->d_q.ensure(n * 8)); HIPCK(plan->d_qin.ensure(n * 8));
     HIPCK(plan->d_p.ensure(n * 8)); HIPCK(plan->d_gx.ensure(n * 8)); HIPCK(plan->d_gy.ensure(n * 8)); HIPCK(plan->d_gz.ensure(n * 8));
+    HIPCK(plan->d_pn.ensure(n * 8)); HIPCK(plan->d_gxn.ensure(n * 8)); HIPCK(plan->d_gyn.ensure(n * 8));
+    HIPCK(plan->d_gzn.ensure(n * 8));
     HIPCK(plan->d_phi.ensure(n * 8)); HIPCK(plan->d_grad.ensure(n * 24));
     HIPCK(plan->d_pcleaf.ensure((size_t)g.nleaf * Sn * 8));
     HIPCK(plan->d_pcleaf2.ensure((size_t)g.nleaf * Sn * 8));
@@ -162,9 +169,12 @@ int ensure_host_structure(fmm3dpp_plan* plan) {
     return FMM3DPP_OK;
 }
 
+int launch_near(fmm3dpp_plan* plan);
+
 // Far field of an evaluation (fmm.cpp:50-74 up to L2P): gather, P2M, M2M, M2L, L2L, L2P, on the
-// plan's stream.  Needs the tree (build_tree), not the near-field lists.
-int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
+// plan's stream.  Needs the tree (build_tree), not the near-field lists.  near_now: the lists are
+// built, so the P2P starts on the side stream as soon as the charges are gathered.
+int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     hipStream_t s = plan->stream;
     const Operators& o = plan->ops;
     const GpuStructure& g = plan->gs;
@@ -179,6 +189,11 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
 
     HIPCK(hipEventRecord(plan->ev[0], s));
     HIPCK(launch_gather(d_qin, g.perm.as<int32_t>(), plan->d_q.as<double>(), n, s));
+    HIPCK(hipEventRecord(plan->ev[kEvFork], s));
+    if (near_now) {
+        int rc = launch_near(plan);
+        if (rc) return rc;
+    }
     HIPCK(hipMemsetAsync(pck, 0, nc * Sn * sizeof(double), s));
     // P2M (fmm.cpp:83-109): check potentials, then q_equiv = 0.5^depth * VSinv * (UT * p_check)
     HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
@@ -215,8 +230,8 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
     if (!plan->mixed()) {
         HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), plan->dop.o_gk8.p, plan->d_Qk.p,
-                               plan->d_Pk.p, s));
+        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
+                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                 plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
@@ -225,8 +240,8 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
         // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
         HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
         HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), plan->dop.o_gk8f.p, plan->d_Qk.p,
-                               plan->d_Pk.p, s));
+        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
+                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
         HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
         if (nimg > 0) {
@@ -265,39 +280,50 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin) {
                           plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
                           plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
     HIPCK(hipEventRecord(plan->ev[7], s));
-    HIPCK(hipEventRecord(plan->ev[kStages + 1], s));
     return FMM3DPP_OK;
 }
 
-// Near field and the final correction (fmm.cpp:75-80): P2P over the lists, dipole term, 1/(4 pi),
-// input order.  Needs build_lists.
-int eval_near(fmm3dpp_plan* plan, double* d_phi, double* d_grad) {
-    hipStream_t s = plan->stream;
+// Near field (P2P, fmm.cpp:259-294) on the side stream, from the point where the charges are in
+// tree order (kEvFork); stored apart from the far field, which the finalize adds.  Needs the lists.
+int launch_near(fmm3dpp_plan* plan) {
+    hipStream_t s = plan->side_stream;
     const GpuStructure& g = plan->gs;
-    const int n = (int)g.n, nleaf = g.nleaf;
+    const int nleaf = g.nleaf;
     const CellsDev cells = plan->cells_dev();
     const BodiesDev bod = plan->bodies_dev();
-    HIPCK(hipEventRecord(plan->ev[7], s));
-    // P2P (fmm.cpp:259-294)
+    double *pn = plan->d_pn.as<double>(), *gxn = plan->d_gxn.as<double>(), *gyn = plan->d_gyn.as<double>(),
+           *gzn = plan->d_gzn.as<double>();
+    HIPCK(hipStreamWaitEvent(s, plan->ev[kEvFork], 0));
+    HIPCK(hipEventRecord(plan->ev[kEvNear0], s));
     if (!plan->mixed()) {
         HIPCK(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(),
-                         g.p2p_ent.as<int32_t>(), plan->cfg.cycle, plan->d_p.as<double>(), plan->d_gx.as<double>(),
-                         plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
+                         g.p2p_ent.as<int32_t>(), plan->cfg.cycle, pn, gxn, gyn, gzn, false, s));
     } else {
         HIPCK(launch_pack_rel(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, plan->d_rel.as<float4>(), s));
         HIPCK(launch_p2p_f32(g.leaf_cell.as<int32_t>(), nleaf, cells, plan->d_rel.as<float4>(),
-                             g.p2p_off.as<int64_t>(), g.p2p_ent.as<int32_t>(), plan->cfg.cycle,
-                             plan->d_p.as<double>(), plan->d_gx.as<double>(), plan->d_gy.as<double>(),
-                             plan->d_gz.as<double>(), s));
+                             g.p2p_off.as<int64_t>(), g.p2p_ent.as<int32_t>(), plan->cfg.cycle, pn, gxn, gyn, gzn,
+                             false, s));
     }
+    HIPCK(hipEventRecord(plan->ev[kEvNear1], s));
+    return FMM3DPP_OK;
+}
+
+// Join of the near field, dipole correction, 1/(4 pi), input order (fmm.cpp:75-80).
+int eval_final(fmm3dpp_plan* plan, double* d_phi, double* d_grad) {
+    hipStream_t s = plan->stream;
+    const GpuStructure& g = plan->gs;
+    const int n = (int)g.n;
+    const BodiesDev bod = plan->bodies_dev();
+    HIPCK(hipStreamWaitEvent(s, plan->ev[kEvNear1], 0));
     HIPCK(hipEventRecord(plan->ev[8], s));
-    // dipole correction, 1/(4 pi), input order (fmm.cpp:75-80)
     const int nblk = 256;
+    const NearDev near{plan->d_pn.as<double>(), plan->d_gxn.as<double>(), plan->d_gyn.as<double>(),
+                       plan->d_gzn.as<double>()};
     HIPCK(launch_dipole_partial(bod, n, plan->d_partial.as<double>(), nblk, s));
-    HIPCK(launch_finalize(n, g.perm.as<int32_t>(), bod, plan->d_p.as<double>(),
-                          plan->d_gx.as<double>(), plan->d_gy.as<double>(), plan->d_gz.as<double>(),
-                          plan->d_partial.as<double>(), nblk, plan->cfg.dipole_mode, plan->cfg.cycle,
-                          plan->cfg.divide_4pi ? 1.0 / (4 * M_PI) : 1.0, d_phi, d_grad, s));
+    HIPCK(launch_finalize(n, g.perm.as<int32_t>(), bod, plan->d_p.as<double>(), plan->d_gx.as<double>(),
+                          plan->d_gy.as<double>(), plan->d_gz.as<double>(), near, plan->d_partial.as<double>(), nblk,
+                          plan->cfg.dipole_mode, plan->cfg.cycle, plan->cfg.divide_4pi ? 1.0 / (4 * M_PI) : 1.0,
+                          d_phi, d_grad, s));
     HIPCK(hipEventRecord(plan->ev[9], s));
     plan->ev_recorded = true;
     return FMM3DPP_OK;
@@ -323,9 +349,9 @@ int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** ou
         if (hipSetDevice(device) != hipSuccess) return FMM3DPP_ERR_DEVICE;
         if (hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
         plan->own_stream = true;
-        for (int i = 0; i <= kStages + 1; i++)
+        for (int i = 0; i < kEvents; i++)
             if (hipEventCreate(&plan->ev[i]) != hipSuccess) return FMM3DPP_ERR_HIP;
-        if (hipStreamCreateWithFlags(&plan->list_stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
+        if (hipStreamCreateWithFlags(&plan->side_stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
     }
     *out = plan.release();
     return FMM3DPP_OK;
@@ -336,12 +362,18 @@ int fmm3dpp_plan_destroy(fmm3dpp_plan* plan) {
     if (plan->device >= 0) {
         (void)hipSetDevice(plan->device);
         if (plan->stream) (void)hipStreamSynchronize(plan->stream);
-        if (plan->list_stream) (void)hipStreamSynchronize(plan->list_stream);
-        for (int i = 0; i <= kStages + 1; i++) if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
+        if (plan->side_stream) (void)hipStreamSynchronize(plan->side_stream);
+        for (int i = 0; i < kEvents; i++) if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
         if (plan->own_stream && plan->stream) (void)hipStreamDestroy(plan->stream);
-        if (plan->list_stream) (void)hipStreamDestroy(plan->list_stream);
+        if (plan->side_stream) (void)hipStreamDestroy(plan->side_stream);
     }
     delete plan;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_set_hadamard_staging(fmm3dpp_plan* plan, int enable) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    plan->had_cap = enable ? kHadCap : -1;
     return FMM3DPP_OK;
 }
 
@@ -413,9 +445,9 @@ int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_qin, double* d_p
     HIPCK(hipSetDevice(plan->device));
     int rc = ensure_ops(plan);
     if (rc) return rc;
-    rc = eval_far(plan, d_qin);
+    rc = eval_far(plan, d_qin, true);
     if (rc) return rc;
-    return eval_near(plan, d_phi, d_grad);
+    return eval_final(plan, d_phi, d_grad);
 }
 
 int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const double* d_y, const double* d_z,
@@ -433,13 +465,15 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
     rc = build_device(plan, n, d_x, d_y, d_z, false);   // tree on the plan's stream, completed on return
     if (rc) return rc;
     plan->host_ms["tree"] = now_ms() - t0;
-    rc = eval_far(plan, d_q);                           // queued on the plan's stream
+    rc = eval_far(plan, d_q, false);                    // queued on the plan's stream
     if (rc) return rc;
-    rc = plan->gs.build_lists(plan->cfg, plan->list_stream, plan->err);   // host waits on the list stream only
+    rc = plan->gs.build_lists(plan->cfg, plan->side_stream, plan->err);   // host waits on the side stream only
     if (rc) return rc;
     plan->host_ms.erase("tree_and_lists");   // the lists overlap the far field: no serial figure
     plan->particles = true;
-    return eval_near(plan, d_phi, d_grad);
+    rc = launch_near(plan);
+    if (rc) return rc;
+    return eval_final(plan, d_phi, d_grad);
 }
 
 int fmm3dpp_evaluate(fmm3dpp_plan* plan, const double* q, double* phi, double* grad) {
@@ -600,7 +634,7 @@ double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
         return hipEventElapsedTime(&ms, p->ev[a], p->ev[b]) == hipSuccess ? (double)ms : -1.0;
     };
     if (s == "evaluate") return el(0, kStages);
-    if (s == "L2P") return el(6, kStages + 1);   // the near field may start later (fmm3dpp_solve_device)
+    if (s == "P2P") return el(kEvNear0, kEvNear1);   // on the side stream, beside the far field
     for (int i = 0; i < kStages; i++)
         if (s == kStageNames[i]) return el(i, i + 1);
     return -1.0;

@@ -87,7 +87,9 @@ int fmm3dpp_set_particles_device(fmm3dpp_plan* plan, int64_t n, const double* d_
 int fmm3dpp_evaluate(fmm3dpp_plan* plan, const double* q, double* phi, double* grad);
 
 /* Same as fmm3dpp_evaluate with device pointers, ordered on the plan's stream (no host sync
- * other than what the tree build needs).  d_grad is n x 3 row-major. */
+ * other than what the tree build needs).  d_grad is n x 3 row-major.  The P2P runs on a second
+ * stream of the plan, forked from the plan's stream once the charges are read and joined back
+ * before the final correction, so it overlaps the far field; callers see stream order. */
 int fmm3dpp_evaluate_device(fmm3dpp_plan* plan, const double* d_q, double* d_phi, double* d_grad);
 
 /* One whole step on device arrays, LaplaceFMM::solve (src/fmm.cpp:13-81) without host copies:
@@ -100,6 +102,12 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
 
 /* Sets the HIP stream (hipStream_t) the plan launches on; NULL = the default stream. */
 int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* hip_stream);
+
+/* No counterpart in the reference (a measurement switch of this implementation).  The 8x8 M2L
+ * Hadamard copies the distinct source spectra of each chunk of 256 targets into LDS before the
+ * matrix ops (enable = 1, the default); enable = 0 reads every source vector from HBM / L2 instead.
+ * Results are bit-identical either way. */
+int fmm3dpp_set_hadamard_staging(fmm3dpp_plan* plan, int enable);
 
 /* ---- structure, for bit-exact comparison with the reference's tree and lists ---- */
 typedef struct fmm3dpp_counts {
@@ -212,8 +220,9 @@ double fmm3dpp_batch_timing_ms(const fmm3dpp_batch* batch, const char* name);
 
 /* Stage timings of the last evaluate in milliseconds (HIP events on the plan's stream); names:
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
- * L2L, L2P, P2P, finalize, evaluate (for solve_device: from the first far-field launch, so it
- * includes the list build that overlaps it); host wall times: tree_and_lists (last
+ * L2L, L2P, P2P (on the second stream, overlapping the far-field stages), finalize (from the join
+ * of the near field), evaluate (for solve_device: from the first far-field launch, so it includes
+ * the list build that overlaps it); host wall times: tree_and_lists (last
  * set_particles), tree (last solve_device, the tree build before the far field starts),
  * precompute.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);

@@ -45,3 +45,39 @@ def test_solve_device_equals_two_calls(precision):
         print(f"  max |diff| phi {dp:.3e} grad {dg:.3e}", flush=True)
         assert dp == 0.0 and dg == 0.0
         assert plan.counts() == ref.counts()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_near_field_stream_order(precision):
+    """The P2P runs on a second stream of the plan, forked from and joined back to the plan's
+    stream.  Two evaluations queued back to back (no host sync between them) on a caller's stream
+    equal the same evaluations run one at a time on a plan's own stream, bit for bit."""
+    import torch
+    import pyfmm3dpp
+    dev = torch.device("cuda", 0)
+    n = 30000
+    x, q1 = _case(n, 11, True)
+    q2 = np.random.Generator(np.random.PCG64(12)).uniform(-1, 1, n)
+    d = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(dev) for k in range(3)]
+    dq = [torch.from_numpy(q).to(dev) for q in (q1, q2)]
+    out = [(torch.empty(n, dtype=torch.float64, device=dev), torch.empty((n, 3), dtype=torch.float64, device=dev))
+           for _ in range(4)]
+    mk = lambda: pyfmm3dpp.LaplaceFMM(5, 40, 2, 1.0, precision=precision, device=0)
+    a, b = mk(), mk()
+    stream = torch.cuda.Stream(device=dev)
+    b.set_stream(stream.cuda_stream)
+    torch.cuda.synchronize()
+    for p in (a, b):
+        p.set_particles_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+    for k in range(2):   # one at a time on the plan's own stream
+        a.evaluate_device(dq[k].data_ptr(), out[k][0].data_ptr(), out[k][1].data_ptr())
+        torch.cuda.synchronize()
+    for k in range(2):   # back to back on the caller's stream
+        b.evaluate_device(dq[k].data_ptr(), out[2 + k][0].data_ptr(), out[2 + k][1].data_ptr())
+    stream.synchronize()
+    for k in range(2):
+        dp = float((out[k][0] - out[2 + k][0]).abs().max())
+        dg = float((out[k][1] - out[2 + k][1]).abs().max())
+        print(f"evaluation {k}: max |diff| phi {dp:.3e} grad {dg:.3e}", flush=True)
+        assert dp == 0.0 and dg == 0.0
+    assert float(out[0][0].abs().max()) > 0 and not torch.equal(out[0][0], out[1][0])
