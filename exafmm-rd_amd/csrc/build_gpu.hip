@@ -99,7 +99,7 @@ __global__ void k_keys(int64_t n, const double* __restrict__ x, const double* __
 // Split of the cells of level l (tree.cpp:147-186): 8 lanes per cell, lane o finds the end of
 // octant o in the cell's key-sorted bodies by its own binary search on digit l, so the search depth
 // is log2(bodies) rather than 8 x that.  cstart[9 * lc + o] = first body of octant o, [8] = end.
-__global__ void k_split(int ls, int le, int l, int ncrit, const uint64_t* __restrict__ K1,
+__global__ void k_split(int ls, int le, int l, int ncrit, int dmax, const uint64_t* __restrict__ K1,
                         const int32_t* __restrict__ boff, const int32_t* __restrict__ bnum, int32_t* __restrict__ nchild,
                         int32_t* __restrict__ cstart, int* __restrict__ flag, int32_t* __restrict__ lvl_nl) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,8 +108,8 @@ __global__ void k_split(int ls, int le, int l, int ncrit, const uint64_t* __rest
     const bool valid = c64 < le;
     const int c = valid ? (int)c64 : ls;   // lanes past the level keep the shuffles below well defined
     const int lc = c - ls, b = boff[c], m = bnum[c];
-    const bool split = valid && m > ncrit && l < kD;
-    if (valid && m > ncrit && l >= kD && o == 0) atomicOr(flag, 2);
+    const bool split = valid && m > ncrit && l < dmax;   // keys are sorted on their top dmax levels
+    if (valid && m > ncrit && l >= dmax && o == 0) atomicOr(flag, dmax < kD ? 4 : 2);
     int ub = b + m;
     if (split) {
         const int shift = 3 * (kD - l - 1);
@@ -534,18 +534,20 @@ hipError_t scan_incl(DBuf& temp, const T* in, T* out, int64_t n, hipStream_t st)
     return rocprim::inclusive_scan(temp.p, bytes, in, out, (size_t)n, rocprim::plus<T>(), st);
 }
 
-// Stable LSD radix sort over the low `bits` bits.  The onesweep algorithm is used at every size
+// Stable LSD radix sort over bits [begin_bit, bits).  The onesweep algorithm is used at every size
 // above one block (rocPRIM's default switches to a merge sort up to 2^20 items, measured ~2x
 // slower here for the body keys and the list entries).
 using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                               rocprim::default_config, 0>;
 template <class K, class V>
-hipError_t sort_pairs(DBuf& temp, const K* kin, K* kout, const V* vin, V* vout, int64_t n, int bits, hipStream_t st) {
+hipError_t sort_pairs(DBuf& temp, const K* kin, K* kout, const V* vin, V* vout, int64_t n, int bits, hipStream_t st,
+                      int begin_bit = 0) {
     size_t bytes = 0;
-    hipError_t e = rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    hipError_t e = rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, kin, kout, vin, vout, (size_t)n, begin_bit,
+                                                        bits, st);
     if (e != hipSuccess) return e;
     if ((e = temp.ensure(bytes)) != hipSuccess) return e;
-    return rocprim::radix_sort_pairs<SortConfig>(temp.p, bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    return rocprim::radix_sort_pairs<SortConfig>(temp.p, bytes, kin, kout, vin, vout, (size_t)n, begin_bit, bits, st);
 }
 
 template <class T>
@@ -585,11 +587,10 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     k_keys<<<blocks(n), 256, 0, st>>>(n, dx, dy, dz, cfg.center[0], cfg.center[1], cfg.center[2], r0,
                                        key.as<uint64_t>(), idx0.as<int32_t>(), flag);
     CK(hipGetLastError());
-    CK(sort_pairs(temp, key.as<uint64_t>(), key_s.as<uint64_t>(), idx0.as<int32_t>(), perm1.as<int32_t>(), n, 3 * kD, st));
-    int hflag = 0;
-    CK(read_back(flag, &hflag, 1, st));
-    if (hflag & 1) { err = "non-finite body position"; return FMM3DPP_ERR_ARG; }
-    // --- 2. top-down levels
+    // The keys are sorted on their top D levels only (fewer radix passes), D = the previous tree's
+    // depth + 2: the tree needs key bits down to its own depth, and the per-leaf sort below restores
+    // input order inside every leaf whatever order the bodies of one leaf have.  A cell that would
+    // split below level D sets flag bit 4 and the build restarts with all kD levels sorted.
     auto grow_cells = [&](size_t need) -> hipError_t {
         if (need <= cell_cap) return hipSuccess;
         size_t cap = std::max(need + (size_t)images + 8, cell_cap * 2);
@@ -609,43 +610,57 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
         cell_cap = cap;
         return hipSuccess;
     };
-    CK(grow_cells(std::max<size_t>(1024, (size_t)(n / std::max(1, cfg.ncrit)) * 3)));
-    k_root_init<<<1, 1, 0, st>>>(cfg.center[0], cfg.center[1], cfg.center[2], r0, (int32_t)n, prefix.as<uint64_t>(),
-                                 cx.as<double>(), cy.as<double>(), cz.as<double>(), cr.as<double>(),
-                                 depth.as<int32_t>(), octant.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
-                                 coff.as<int32_t>(), cnum.as<int32_t>());
-    CK(hipGetLastError());
-    CK(lvl_nl.ensure((kD + 2) * 4));
-    CK(hipMemsetAsync(lvl_nl.p, 0, (kD + 2) * 4, st));
-    level_start.assign(1, 0);
-    level_start.push_back(1);
-    for (int l = 0;; l++) {
-        int ls = level_start[l], le = level_start[l + 1], ncl = le - ls;
-        CK(nchild.ensure((size_t)ncl * 4)); CK(scan32.ensure((size_t)ncl * 4)); CK(cstart.ensure((size_t)ncl * 9 * 4));
-        k_split<<<blocks((int64_t)ncl * 8), 256, 0, st>>>(ls, le, l, cfg.ncrit, key_s.as<uint64_t>(), boff.as<int32_t>(),
-                                              bnum.as<int32_t>(), nchild.as<int32_t>(), cstart.as<int32_t>(), flag,
-                                              lvl_nl.as<int32_t>());
+    int D = sort_levels > 0 && sort_levels < kD ? sort_levels : kD;
+    int hflag = 0;
+    for (bool retry = false;; retry = true) {
+        if (retry) CK(hipMemsetAsync(small.p, 0, 64, st));   // bit 4 of the first attempt
+        CK(sort_pairs(temp, key.as<uint64_t>(), key_s.as<uint64_t>(), idx0.as<int32_t>(), perm1.as<int32_t>(), n,
+                      3 * kD, st, 3 * (kD - D)));
+        // --- 2. top-down levels
+        CK(grow_cells(std::max<size_t>(1024, (size_t)(n / std::max(1, cfg.ncrit)) * 3)));
+        k_root_init<<<1, 1, 0, st>>>(cfg.center[0], cfg.center[1], cfg.center[2], r0, (int32_t)n, prefix.as<uint64_t>(),
+                                     cx.as<double>(), cy.as<double>(), cz.as<double>(), cr.as<double>(),
+                                     depth.as<int32_t>(), octant.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>(),
+                                     coff.as<int32_t>(), cnum.as<int32_t>());
         CK(hipGetLastError());
-        CK(scan_incl(temp, nchild.as<int32_t>(), scan32.as<int32_t>(), ncl, st));
-        k_copy_last<<<1, 1, 0, st>>>(scan32.as<int32_t>(), ncl - 1, flag);
-        CK(hipGetLastError());
-        int32_t fl[2] = {0, 0};   // error flag, children total
-        CK(read_back(flag, fl, 2, st));
-        hflag = fl[0];
-        const int32_t total = fl[1];
-        if (hflag & 2) { err = "more than ncrit bodies share a cell at depth 21 (coincident bodies?)"; return FMM3DPP_ERR_TREE; }
-        CK(grow_cells((size_t)le + total));
-        k_emit_children<<<blocks(ncl), 256, 0, st>>>(ls, le, nchild.as<int32_t>(), scan32.as<int32_t>(),
-                                                     cstart.as<int32_t>(), prefix.as<uint64_t>(), cx.as<double>(),
-                                                     cy.as<double>(), cz.as<double>(), cr.as<double>(),
-                                                     depth.as<int32_t>(), octant.as<int32_t>(), coff.as<int32_t>(),
-                                                     cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>());
-        CK(hipGetLastError());
-        if (total == 0) break;
-        level_start.push_back(le + total);
+        CK(lvl_nl.ensure((kD + 2) * 4));
+        CK(hipMemsetAsync(lvl_nl.p, 0, (kD + 2) * 4, st));
+        level_start.assign(1, 0);
+        level_start.push_back(1);
+        bool deeper = false;
+        for (int l = 0;; l++) {
+            int ls = level_start[l], le = level_start[l + 1], ncl = le - ls;
+            CK(nchild.ensure((size_t)ncl * 4)); CK(scan32.ensure((size_t)ncl * 4)); CK(cstart.ensure((size_t)ncl * 9 * 4));
+            k_split<<<blocks((int64_t)ncl * 8), 256, 0, st>>>(ls, le, l, cfg.ncrit, D, key_s.as<uint64_t>(),
+                                                              boff.as<int32_t>(), bnum.as<int32_t>(), nchild.as<int32_t>(),
+                                                              cstart.as<int32_t>(), flag, lvl_nl.as<int32_t>());
+            CK(hipGetLastError());
+            CK(scan_incl(temp, nchild.as<int32_t>(), scan32.as<int32_t>(), ncl, st));
+            k_copy_last<<<1, 1, 0, st>>>(scan32.as<int32_t>(), ncl - 1, flag);
+            CK(hipGetLastError());
+            int32_t fl[2] = {0, 0};   // error flag, children total
+            CK(read_back(flag, fl, 2, st));
+            hflag = fl[0];
+            const int32_t total = fl[1];
+            if (hflag & 1) { err = "non-finite body position"; return FMM3DPP_ERR_ARG; }
+            if (hflag & 4) { deeper = true; break; }
+            if (hflag & 2) { err = "more than ncrit bodies share a cell at depth 21 (coincident bodies?)"; return FMM3DPP_ERR_TREE; }
+            CK(grow_cells((size_t)le + total));
+            k_emit_children<<<blocks(ncl), 256, 0, st>>>(ls, le, nchild.as<int32_t>(), scan32.as<int32_t>(),
+                                                         cstart.as<int32_t>(), prefix.as<uint64_t>(), cx.as<double>(),
+                                                         cy.as<double>(), cz.as<double>(), cr.as<double>(),
+                                                         depth.as<int32_t>(), octant.as<int32_t>(), coff.as<int32_t>(),
+                                                         cnum.as<int32_t>(), boff.as<int32_t>(), bnum.as<int32_t>());
+            CK(hipGetLastError());
+            if (total == 0) break;
+            level_start.push_back(le + total);
+        }
+        if (!deeper) break;
+        D = kD;   // a deeper tree than the sorted levels: sort all levels and build again
     }
     ntree = level_start.back();
     maxd = (int)level_start.size() - 2;
+    sort_levels = std::min(kD, maxd + 2);
     ncells = ntree + images;
     CK(grow_cells((size_t)ncells));
     // --- 3. leaf depth, stable re-sort, positions in tree order, leaves in body order

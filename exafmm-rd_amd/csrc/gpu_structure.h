@@ -18,6 +18,7 @@ struct GpuStructure {
     // sizes (host mirror)
     int64_t n = 0;
     int ntree = 0, ncells = 0, nleaf = 0, maxd = 0, nl = 0, nimg = 0, ntar = 0;
+    int sort_levels = 0;   // key levels the next build sorts first (the last tree's depth + 2; 0: all)
     int64_t n_rows = 0, n_entries = 0, lost = 0;
     int max_level_cols = 1;
     std::vector<int32_t> level_start;          // tree levels 0..maxd, plus end

@@ -157,50 +157,42 @@ __global__ __launch_bounds__(256) void k_hadamard_lanes(int F, const T* __restri
 }
 
 // Staging tables of the Hadamard, one workgroup per chunk c of kHadChunk consecutive targets: the
-// distinct sources of all 26 patterns of the chunk in increasing order (uniq[c * kHadUniqStride + r],
-// r < cnt[c]) and, per (pattern, target), the row of its source among them
-// (loc[(c * 26 + p) * kHadChunk + tt]; cnt[c] when there is none).  Bitonic sort of the chunk's
-// 26 x kHadChunk source indices in LDS, unique marking, a block scan for the ranks, then a binary
-// search per entry.
-constexpr int kHadKeys = 8192;   // 26 * kHadChunk rounded up to a power of two
-static_assert(26 * kHadChunk <= kHadKeys, "sort size");
+// distinct sources of all 26 patterns of the chunk (uniq[c * kHadUniqStride + r], r < cnt[c]) and,
+// per (pattern, target), the row of its source among them (loc[(c * 26 + p) * kHadChunk + tt];
+// cnt[c] when there is none).  The chunk's source indices are inserted into an open-addressing hash
+// set in LDS (linear probing, more slots than entries, so every probe sequence ends), the occupied
+// slots are ranked by a block scan, and each entry looks its source up again.  The row order
+// depends on the insertion order; the Hadamard's results do not depend on the rows.
+constexpr int kHadHash = 8192;   // hash slots, a power of two above 26 * kHadChunk
+static_assert(26 * kHadChunk < kHadHash && 26 * kHadChunk < 32768, "hash slots, 16-bit ranks");
 __global__ __launch_bounds__(1024) void k_hadamard_tables(int ntar, int ntab, const int32_t* __restrict__ slot_src_T,
                                                           int32_t* __restrict__ uniq, int32_t* __restrict__ cnt,
                                                           int32_t* __restrict__ loc) {
-    __shared__ int32_t keys[kHadKeys];
-    __shared__ int32_t uq[26 * kHadChunk];
+    __shared__ int32_t hkey[kHadHash];
+    __shared__ int16_t hrank[kHadHash];   // ranks < 26 * kHadChunk < 2^15
     __shared__ int32_t part[1024];
-    constexpr int32_t kNone = 0x7fffffff;
     const int c = blockIdx.x, tid = threadIdx.x;
     auto source = [&](int i) {   // entry i = p * kHadChunk + tt of the chunk, -1 when none
         const int p = i / kHadChunk, t = c * kHadChunk + i % kHadChunk;
         return t < ntar ? slot_src_T[(size_t)p * ntab + t] : -1;
     };
-    for (int i = tid; i < kHadKeys; i += 1024) {
-        const int s = i < 26 * kHadChunk ? source(i) : -1;
-        keys[i] = s < 0 ? kNone : s;
+    auto slot0 = [](int32_t s) { return (int)(((uint32_t)s * 2654435761u) >> 19) & (kHadHash - 1); };
+    for (int i = tid; i < kHadHash; i += 1024) hkey[i] = -1;
+    __syncthreads();
+    for (int i = tid; i < 26 * kHadChunk; i += 1024) {
+        const int32_t s = source(i);
+        if (s < 0) continue;
+        int h = slot0(s);
+        for (int probe = 0; probe < kHadHash; probe++) {   // at most kHadHash - 1 other keys
+            const int32_t old = atomicCAS(&hkey[h], -1, s);
+            if (old == -1 || old == s) break;
+            h = (h + 1) & (kHadHash - 1);
+        }
     }
     __syncthreads();
-    for (int k = 2; k <= kHadKeys; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < kHadKeys; i += 1024) {
-                const int ij = i ^ j;
-                if (ij > i) {
-                    const int32_t a = keys[i], b = keys[ij];
-                    if ((a > b) == ((i & k) == 0)) {
-                        keys[i] = b;
-                        keys[ij] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    constexpr int PER = kHadKeys / 1024;   // consecutive keys per thread
+    constexpr int PER = kHadHash / 1024;   // consecutive slots per thread
     int mine = 0;
-    for (int e = 0; e < PER; e++) {
-        const int i = tid * PER + e;
-        mine += keys[i] != kNone && (i == 0 || keys[i] != keys[i - 1]);
-    }
+    for (int e = 0; e < PER; e++) mine += hkey[tid * PER + e] >= 0;
     part[tid] = mine;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan of the per-thread counts
@@ -213,25 +205,21 @@ __global__ __launch_bounds__(1024) void k_hadamard_tables(int ntar, int ntab, co
     int r = part[tid] - mine;
     int32_t* un = uniq + (size_t)c * kHadUniqStride;
     for (int e = 0; e < PER; e++) {
-        const int i = tid * PER + e;
-        if (keys[i] != kNone && (i == 0 || keys[i] != keys[i - 1])) {
-            uq[r] = keys[i];
-            un[r] = keys[i];
+        const int h = tid * PER + e;
+        if (hkey[h] >= 0) {
+            hrank[h] = (int16_t)r;
+            un[r] = hkey[h];
             r++;
         }
     }
     __syncthreads();
     for (int i = tid; i < 26 * kHadChunk; i += 1024) {
-        const int s = source(i);
+        const int32_t s = source(i);
         int l = total;
-        if (s >= 0) {   // s is among uq[0, total)
-            int lo = 0, hi = total - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (uq[mid] < s) lo = mid + 1;
-                else hi = mid;
-            }
-            l = lo;
+        if (s >= 0) {   // s is in the set
+            int h = slot0(s);
+            for (int probe = 0; probe < kHadHash && hkey[h] != s; probe++) h = (h + 1) & (kHadHash - 1);
+            if (hkey[h] == s) l = hrank[h];   // always found; otherwise the zero row
         }
         loc[(size_t)c * 26 * kHadChunk + i] = l;
     }

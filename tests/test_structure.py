@@ -72,3 +72,32 @@ def test_host_structure_matches_reference(oracle_mod, case):
 def test_gpu_structure_matches_reference(oracle_mod, case):
     print(f"gpu structure case {case}", flush=True)
     _compare(oracle_mod, 0, case)
+
+
+@pytest.mark.gpu
+def test_gpu_structure_reused_plan(oracle_mod):
+    """One device plan, several point sets in a row.  After the first build the keys are sorted on
+    the previous tree's depth + 2 levels only; a deeper tree (clustered after uniform) restarts the
+    build with every level sorted.  Every build must equal the restated reference."""
+    import pyfmm3dpp
+    ncrit, images, cycle = 32, 2, 1.0
+    plan = pyfmm3dpp.LaplaceFMM(4, ncrit, images, cycle, device=0)
+    depths = []
+    for kind, n in (("uniform", 4096), ("clustered", 20000), ("uniform", 20000), ("clustered", 20000),
+                    ("uniform", 300)):
+        print(f"reused plan: {kind} n={n}", flush=True)
+        x, y, z = _points(oracle_mod, kind, n, cycle)
+        orc = oracle_mod.OracleFMM(4, ncrit, images, cycle)
+        orc.set_particles(x, y, z)
+        plan.set_particles(x, y, z)
+        oi, od = orc.cells()
+        pi, pd = plan.cells()
+        assert np.array_equal(oi, pi) and np.array_equal(od, pd)
+        assert np.array_equal(orc.body_order(), plan.body_order())
+        assert _rows(*orc.p2p_list()) == _rows(*plan.p2p_list())
+        t1, s1, o1, i1 = orc.m2l_list()
+        t2, s2, o2, i2 = plan.m2l_list()
+        assert _rows(t1, s1, o1, i1) == _rows(t2, s2, o2, i2)
+        depths.append(int(pi[: plan.counts()["n_tree_cells"], 2].max()))
+    print("tree depths", depths, flush=True)
+    assert depths[1] > depths[0] + 2   # the clustered tree is deeper than the levels sorted for it
