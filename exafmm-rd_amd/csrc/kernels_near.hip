@@ -26,6 +26,14 @@ __device__ __forceinline__ double inv_sqrt(double r2) {
     return r2 > 0 ? y : 0.0;
 }
 
+// For r2 > 0 known in advance: bodies of a leaf against its check / equivalent surface, which lies
+// 2.95 r from the centre while the bodies are within sqrt(3) r, so r2 >= (1.95 r)^2.
+__device__ __forceinline__ double inv_sqrt_pos(double r2) {
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double t = r2 * y * y;
+    return y * fma(-0.5, t, 1.5);
+}
+
 __device__ __forceinline__ float inv_sqrt(float r2) {
     const float y = __builtin_amdgcn_rsqf(r2);
     return r2 > 0 ? y : 0.0f;
@@ -241,7 +249,7 @@ template <int KP>
 __global__ __launch_bounds__(64) void k_p2m_check(int S, const double* __restrict__ usurf,
                                                   const int32_t* __restrict__ leaf_cell, CellsDev cells, BodiesDev b,
                                                   double* __restrict__ pc) {
-    __shared__ double sx[64], sy[64], sz[64], sq[64];
+    __shared__ double4 sb[64];   // (x, y, z, q) of 64 bodies
     const int li = blockIdx.x, lane = threadIdx.x;
     const int c = leaf_cell[li];
     const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
@@ -260,18 +268,16 @@ __global__ __launch_bounds__(64) void k_p2m_check(int S, const double* __restric
         const int nb = min(64, bnum - b0);
         __syncthreads();
         if (lane < nb) {
-            sx[lane] = b.x[boff + b0 + lane];
-            sy[lane] = b.y[boff + b0 + lane];
-            sz[lane] = b.z[boff + b0 + lane];
-            sq[lane] = b.q[boff + b0 + lane];
+            const int bj = boff + b0 + lane;
+            sb[lane] = make_double4(b.x[bj], b.y[bj], b.z[bj], b.q[bj]);
         }
         __syncthreads();
         for (int j = 0; j < nb; j++) {
-            const double xj = sx[j], yj = sy[j], zj = sz[j], qj = sq[j];
+            const double4 e = sb[j];
 #pragma unroll
             for (int m = 0; m < KP; m++) {
-                const double dx = px[m] - xj, dy = py[m] - yj, dz = pz[m] - zj;
-                acc[m] += qj * inv_sqrt(dx * dx + dy * dy + dz * dz);
+                const double dx = px[m] - e.x, dy = py[m] - e.y, dz = pz[m] - e.z;
+                acc[m] += e.w * inv_sqrt_pos(dx * dx + dy * dy + dz * dz);
             }
         }
     }
@@ -290,20 +296,13 @@ __global__ __launch_bounds__(64) void k_l2p_eval(int S, const double* __restrict
                                                  const double* __restrict__ qdown, double* __restrict__ p,
                                                  double* __restrict__ gx, double* __restrict__ gy,
                                                  double* __restrict__ gz) {
-    extern __shared__ double lds[];
-    double* ex = lds;
-    double* ey = ex + S;
-    double* ez = ey + S;
-    double* eq = ez + S;
+    extern __shared__ double4 epts[];   // S equivalent points (x, y, z, density)
     const int li = blockIdx.x, lane = threadIdx.x;
     const int c = leaf_cell[li];
     const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
-    for (int s = lane; s < S; s += 64) {
-        ex[s] = usurf[3 * s] * rr + cx;
-        ey[s] = usurf[3 * s + 1] * rr + cy;
-        ez[s] = usurf[3 * s + 2] * rr + cz;
-        eq[s] = qdown[(size_t)li * S + s];
-    }
+    for (int s = lane; s < S; s += 64)
+        epts[s] = make_double4(usurf[3 * s] * rr + cx, usurf[3 * s + 1] * rr + cy, usurf[3 * s + 2] * rr + cz,
+                               qdown[(size_t)li * S + s]);
     __syncthreads();
     const int boff = cells.boff[c], bnum = cells.bnum[c];
     const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
@@ -316,9 +315,10 @@ __global__ __launch_bounds__(64) void k_l2p_eval(int S, const double* __restrict
         double ap = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll 4   // independent evaluations in flight; the sums keep their order
         for (int s = g; s < S; s += G) {
-            const double dx = xt - ex[s], dy = yt - ey[s], dz = zt - ez[s];
-            const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
-            double qi = eq[s] * ir;
+            const double4 e = epts[s];
+            const double dx = xt - e.x, dy = yt - e.y, dz = zt - e.z;
+            const double ir = inv_sqrt_pos(dx * dx + dy * dy + dz * dz);
+            double qi = e.w * ir;
             ap += qi;
             qi *= ir * ir;
             ax += qi * dx;

@@ -44,7 +44,16 @@ struct fmm3dpp_plan {
     GpuStructure gs;           // device tree/lists (device plans)
     std::map<std::string, double> host_ms;
     hipEvent_t ev[kEvents] = {};
-    hipStream_t side_stream = nullptr;   // near field (lists, P2P) beside the far field
+    // Near field (lists, P2P) beside the far field: created at the first evaluation that uses it, so
+    // that a process whose plans keep the near field on their own streams holds one stream per plan
+    // (HIP spreads streams over a few hardware queues; fewer streams, more plans on distinct queues).
+    hipStream_t side_stream = nullptr;
+    bool near_side = true;   // near field on side_stream (else on the plan's stream)
+    hipError_t ensure_near_stream() {
+        if (!near_side || side_stream) return hipSuccess;
+        return hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking);
+    }
+    hipStream_t near_stream() const { return near_side ? side_stream : stream; }
     bool ev_recorded = false;
     int had_cap = kHadCap;   // distinct sources a Hadamard chunk may stage through LDS (fmm3dpp_set_hadamard_staging)
 
@@ -286,7 +295,8 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
 // Near field (P2P, fmm.cpp:259-294) on the side stream, from the point where the charges are in
 // tree order (kEvFork); stored apart from the far field, which the finalize adds.  Needs the lists.
 int launch_near(fmm3dpp_plan* plan) {
-    hipStream_t s = plan->side_stream;
+    HIPCK(plan->ensure_near_stream());
+    hipStream_t s = plan->near_stream();
     const GpuStructure& g = plan->gs;
     const int nleaf = g.nleaf;
     const CellsDev cells = plan->cells_dev();
@@ -351,7 +361,6 @@ int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** ou
         plan->own_stream = true;
         for (int i = 0; i < kEvents; i++)
             if (hipEventCreate(&plan->ev[i]) != hipSuccess) return FMM3DPP_ERR_HIP;
-        if (hipStreamCreateWithFlags(&plan->side_stream, hipStreamNonBlocking) != hipSuccess) return FMM3DPP_ERR_HIP;
     }
     *out = plan.release();
     return FMM3DPP_OK;
@@ -368,6 +377,17 @@ int fmm3dpp_plan_destroy(fmm3dpp_plan* plan) {
         if (plan->side_stream) (void)hipStreamDestroy(plan->side_stream);
     }
     delete plan;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_set_near_field_stream(fmm3dpp_plan* plan, int enable) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->device >= 0) {   // work already queued keeps its order
+        HIPCK(hipSetDevice(plan->device));
+        if (plan->side_stream) HIPCK(hipStreamSynchronize(plan->side_stream));
+        HIPCK(hipStreamSynchronize(plan->stream));
+    }
+    plan->near_side = enable != 0;
     return FMM3DPP_OK;
 }
 
@@ -467,7 +487,8 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
     plan->host_ms["tree"] = now_ms() - t0;
     rc = eval_far(plan, d_q, false);                    // queued on the plan's stream
     if (rc) return rc;
-    rc = plan->gs.build_lists(plan->cfg, plan->side_stream, plan->err);   // host waits on the side stream only
+    HIPCK(plan->ensure_near_stream());
+    rc = plan->gs.build_lists(plan->cfg, plan->near_stream(), plan->err);   // host waits on that stream only
     if (rc) return rc;
     plan->host_ms.erase("tree_and_lists");   // the lists overlap the far field: no serial figure
     plan->particles = true;
@@ -635,6 +656,8 @@ double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name) {
     };
     if (s == "evaluate") return el(0, kStages);
     if (s == "P2P") return el(kEvNear0, kEvNear1);   // on the side stream, beside the far field
+    if (s == "near_start") return el(0, kEvNear0);   // far-field start to P2P start
+    if (s == "far_end") return el(0, 7);             // far-field start to the end of L2P
     for (int i = 0; i < kStages; i++)
         if (s == kStageNames[i]) return el(i, i + 1);
     return -1.0;

@@ -23,7 +23,7 @@ DEVICE_NONE = -1
 EXPORTS = (
     "fmm3dpp_plan_create", "fmm3dpp_plan_destroy", "fmm3dpp_last_error", "fmm3dpp_precompute",
     "fmm3dpp_set_particles", "fmm3dpp_set_particles_device", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
-    "fmm3dpp_solve_device", "fmm3dpp_set_hadamard_staging",
+    "fmm3dpp_solve_device", "fmm3dpp_set_hadamard_staging", "fmm3dpp_set_near_field_stream",
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
@@ -80,6 +80,7 @@ def lib():
         L.fmm3dpp_evaluate_device.argtypes = [vp, vp, vp, vp]
         L.fmm3dpp_set_stream.argtypes = [vp, vp]
         L.fmm3dpp_set_hadamard_staging.argtypes = [vp, ctypes.c_int]
+        L.fmm3dpp_set_near_field_stream.argtypes = [vp, ctypes.c_int]
         L.fmm3dpp_solve_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
         L.fmm3dpp_get_counts.argtypes = [vp, ctypes.POINTER(Counts)]
         L.fmm3dpp_get_cells.argtypes = [vp, _ip, _dp]
@@ -231,6 +232,11 @@ class LaplaceFMM:
 
     def set_stream(self, stream_ptr):
         self._check(lib().fmm3dpp_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def set_near_field_stream(self, enable):
+        """1 (default): the near field runs on the plan's second stream beside the far field; 0: on the
+        plan's stream after it.  Results are bit-identical."""
+        self._check(lib().fmm3dpp_set_near_field_stream(self.h, int(bool(enable))))
 
     def set_hadamard_staging(self, enable):
         """1 (default): the 8x8 Hadamard stages each chunk's distinct sources through LDS; 0: it reads

@@ -103,6 +103,13 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
 /* Sets the HIP stream (hipStream_t) the plan launches on; NULL = the default stream. */
 int fmm3dpp_set_stream(fmm3dpp_plan* plan, void* hip_stream);
 
+/* No counterpart in the reference.  enable = 1 (default): the near field (the P2P, and in
+ * solve_device the list build) runs on a second stream of the plan beside the far field, forked
+ * from and joined back to the plan's stream.  enable = 0: it runs on the plan's stream after the far
+ * field.  Results are bit-identical.  The overlap needs the two streams on different hardware queues
+ * (GPU_MAX_HW_QUEUES, 4 by default); with many live plans in one process, 0 can be faster. */
+int fmm3dpp_set_near_field_stream(fmm3dpp_plan* plan, int enable);
+
 /* No counterpart in the reference (a measurement switch of this implementation).  The 8x8 M2L
  * Hadamard copies the distinct source spectra of each chunk of 256 targets into LDS before the
  * matrix ops (enable = 1, the default); enable = 0 reads every source vector from HBM / L2 instead.
@@ -222,7 +229,8 @@ double fmm3dpp_batch_timing_ms(const fmm3dpp_batch* batch, const char* name);
  * P2M, M2M, M2L_up, M2L_hadamard (the 8x8 kernel only), M2L_down (image-lattice 1x27 + down),
  * L2L, L2P, P2P (on the second stream, overlapping the far-field stages), finalize (from the join
  * of the near field), evaluate (for solve_device: from the first far-field launch, so it includes
- * the list build that overlaps it); host wall times: tree_and_lists (last
+ * the list build that overlaps it), near_start / far_end (from the first far-field launch to the
+ * start of the P2P / the end of L2P); host wall times: tree_and_lists (last
  * set_particles), tree (last solve_device, the tree build before the far field starts),
  * precompute.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);

@@ -73,10 +73,16 @@ def _case(fmm, oracle_mod, x, y, z, q, P, ncrit, images, cycle, dipole, div4pi=0
     return plan, phi, grad
 
 
-@pytest.mark.parametrize("P,images", [(4, 0), (4, 1), (6, 2), (8, 3)])
+# The pseudo-inverses amplify rounding along their small singular directions, and their condition
+# number grows with P (3.6e12 at P=8): two float64 evaluations that round in different orders agree
+# to 1e-12 up to P=8 and to 2-4e-12 at P=10 (measured 1.6e-12 potential, 3.7e-12 gradient).
+P_TOL = {10: 1e-11}
+
+
+@pytest.mark.parametrize("P,images", [(2, 1), (4, 0), (4, 1), (6, 2), (8, 3), (10, 1)])
 def test_fp64_matches_oracle_small(fmm, oracle_mod, P, images):
     x, y, z, q = _uniform(4096, 1000 + P + images)
-    _case(fmm, oracle_mod, x, y, z, q, P, 64, images, 1.0, dipole=1)
+    _case(fmm, oracle_mod, x, y, z, q, P, 64, images, 1.0, dipole=1, tol=P_TOL.get(P, 1e-12))
 
 
 def test_fp64_adaptive_clustered_tree(fmm, oracle_mod):

@@ -81,3 +81,40 @@ def test_near_field_stream_order(precision):
         print(f"evaluation {k}: max |diff| phi {dp:.3e} grad {dg:.3e}", flush=True)
         assert dp == 0.0 and dg == 0.0
     assert float(out[0][0].abs().max()) > 0 and not torch.equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_near_field_stream_switch(precision):
+    """fmm3dpp_set_near_field_stream(plan, 0) runs the near field on the plan's own stream: the same
+    kernels in another order, so evaluate_device and solve_device give the same bits."""
+    import torch
+    import pyfmm3dpp
+    dev = torch.device("cuda", 0)
+    n = 30000
+    x, q = _case(n, 21, False)
+    d = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(dev) for k in range(3)]
+    dq = torch.from_numpy(q).to(dev)
+    res = []
+    for side in (1, 0):
+        plan = pyfmm3dpp.LaplaceFMM(6, 40, 2, 1.0, precision=precision, device=0)
+        plan.set_near_field_stream(side)
+        out = []
+        for fused in (False, True):
+            phi = torch.empty(n, dtype=torch.float64, device=dev)
+            grad = torch.empty((n, 3), dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            if fused:
+                plan.solve_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), dq.data_ptr(), phi.data_ptr(),
+                                  grad.data_ptr())
+            else:
+                plan.set_particles_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+                plan.evaluate_device(dq.data_ptr(), phi.data_ptr(), grad.data_ptr())
+            torch.cuda.synchronize()
+            out.append((phi, grad))
+        res.append(out)
+        plan.close()
+    for k in range(2):
+        for a, b in zip(res[0][k], res[1][k]):
+            print(f"max |diff| {float((a - b).abs().max()):.3e}", flush=True)
+            assert torch.equal(a, b)
+        assert torch.equal(res[0][0][0], res[0][1][0])
