@@ -28,8 +28,10 @@ namespace {
 const char* kStageNames[] = {"P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize"};
 constexpr int kStages = 9;
 // Events: 0..kStages are the stage boundaries on the plan's stream (7: end of L2P, 8: near field
-// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream; kEvFork: charges in tree order.
-constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvFork = kStages + 3, kEvents = kStages + 4;
+// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream.  The P2P is forked at ev[4],
+// the end of the 8x8 Hadamard, so it overlaps the down transform, L2L and L2P rather than the
+// Hadamard (both would slow each other down by about the P2P's own time).
+constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvents = kStages + 3, kEvForkNear = 4;
 }  // namespace
 
 struct fmm3dpp_plan {
@@ -182,7 +184,7 @@ int launch_near(fmm3dpp_plan* plan);
 
 // Far field of an evaluation (fmm.cpp:50-74 up to L2P): gather, P2M, M2M, M2L, L2L, L2P, on the
 // plan's stream.  Needs the tree (build_tree), not the near-field lists.  near_now: the lists are
-// built, so the P2P starts on the side stream as soon as the charges are gathered.
+// built, so the P2P is queued on the side stream right after the 8x8 Hadamard.
 int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     hipStream_t s = plan->stream;
     const Operators& o = plan->ops;
@@ -198,11 +200,6 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
 
     HIPCK(hipEventRecord(plan->ev[0], s));
     HIPCK(launch_gather(d_qin, g.perm.as<int32_t>(), plan->d_q.as<double>(), n, s));
-    HIPCK(hipEventRecord(plan->ev[kEvFork], s));
-    if (near_now) {
-        int rc = launch_near(plan);
-        if (rc) return rc;
-    }
     HIPCK(hipMemsetAsync(pck, 0, nc * Sn * sizeof(double), s));
     // P2M (fmm.cpp:83-109): check potentials, then q_equiv = 0.5^depth * VSinv * (UT * p_check)
     HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
@@ -242,6 +239,10 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
                                plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
+        if (near_now) {
+            int rc = launch_near(plan);
+            if (rc) return rc;
+        }
         HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                 plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
         HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
@@ -252,6 +253,10 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
                                plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
+        if (near_now) {
+            int rc = launch_near(plan);
+            if (rc) return rc;
+        }
         HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
         if (nimg > 0) {
             HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
@@ -292,8 +297,9 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     return FMM3DPP_OK;
 }
 
-// Near field (P2P, fmm.cpp:259-294) on the side stream, from the point where the charges are in
-// tree order (kEvFork); stored apart from the far field, which the finalize adds.  Needs the lists.
+// Near field (P2P, fmm.cpp:259-294) on the side stream, from the end of the 8x8 Hadamard on
+// (ev[kEvForkNear], recorded after the charges are in tree order); stored apart from the far field,
+// which the finalize adds.  Needs the lists.
 int launch_near(fmm3dpp_plan* plan) {
     HIPCK(plan->ensure_near_stream());
     hipStream_t s = plan->near_stream();
@@ -303,7 +309,7 @@ int launch_near(fmm3dpp_plan* plan) {
     const BodiesDev bod = plan->bodies_dev();
     double *pn = plan->d_pn.as<double>(), *gxn = plan->d_gxn.as<double>(), *gyn = plan->d_gyn.as<double>(),
            *gzn = plan->d_gzn.as<double>();
-    HIPCK(hipStreamWaitEvent(s, plan->ev[kEvFork], 0));
+    HIPCK(hipStreamWaitEvent(s, plan->ev[kEvForkNear], 0));
     HIPCK(hipEventRecord(plan->ev[kEvNear0], s));
     if (!plan->mixed()) {
         HIPCK(launch_p2p(g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.p2p_off.as<int64_t>(),
