@@ -342,6 +342,19 @@ __device__ inline bool adjacent_d(const CellView& C, int a, int b, double ox, do
            fabs(C.z[a] - C.z[b] - oz) <= dist;
 }
 
+// The traversal's first items: (root, root) with parents `top` (the root, or the outermost image
+// cell) under offset 13 (none) alone without images, else under each of the 27 offsets.
+__global__ void k_frontier_init(int nitems, int top, int32_t* __restrict__ ftc, int32_t* __restrict__ fsc,
+                                int32_t* __restrict__ ftcp, int32_t* __restrict__ fscp, int32_t* __restrict__ foi) {
+    const int i = threadIdx.x;
+    if (i >= nitems) return;
+    ftc[i] = 0;
+    fsc[i] = 0;
+    ftcp[i] = top;
+    fscp[i] = top;
+    foi[i] = nitems == 1 ? 13 : i;
+}
+
 __global__ void k_frontier_classify(int64_t nitems, const int32_t* __restrict__ ftc, const int32_t* __restrict__ fsc,
                                     const int32_t* __restrict__ ftcp, const int32_t* __restrict__ fscp,
                                     const int32_t* __restrict__ foi, CellView C, double cycle,
@@ -550,6 +563,26 @@ hipError_t sort_pairs(DBuf& temp, const K* kin, K* kout, const V* vin, V* vout, 
     return rocprim::radix_sort_pairs<SortConfig>(temp.p, bytes, kin, kout, vin, vout, (size_t)n, begin_bit, bits, st);
 }
 
+// Host table -> device through the build's pinned staging region, in stream order (no host wait).
+template <class T>
+hipError_t upload_staged(GpuStructure::PinnedStage& ps, DBuf& d, const std::vector<T>& h, hipStream_t st) {
+    const size_t bytes = h.size() * sizeof(T), span = (bytes + 255) & ~(size_t)255;
+    hipError_t e = d.ensure(bytes);
+    if (e != hipSuccess || h.empty()) return e;
+    if (ps.off + span > ps.cap) {
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;   // copies in flight read the old region
+        if (ps.p) (void)hipHostFree(ps.p);
+        ps.p = nullptr;
+        ps.cap = std::max(2 * (ps.off + span), (size_t)1 << 16);
+        if ((e = hipHostMalloc((void**)&ps.p, ps.cap)) != hipSuccess) { ps.cap = 0; return e; }
+        ps.off = 0;
+    }
+    std::memcpy(ps.p + ps.off, h.data(), bytes);
+    e = hipMemcpyAsync(d.p, ps.p + ps.off, bytes, hipMemcpyHostToDevice, st);
+    ps.off += span;
+    return e;
+}
+
 template <class T>
 hipError_t read_back(const T* d, T* h, size_t cnt, hipStream_t st) {
     hipError_t e = hipMemcpyAsync(h, d, cnt * sizeof(T), hipMemcpyDeviceToHost, st);
@@ -575,6 +608,7 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
 int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double* dx, const double* dy,
                              const double* dz, hipStream_t st, std::string& err) {
     t_build = std::chrono::steady_clock::now();
+    stage.off = 0;   // the previous build ended with its stream synchronised
     if (nb < 1 || nb > (int64_t)INT32_MAX / 2) { err = "number of bodies must be in [1, 2^30)"; return FMM3DPP_ERR_ARG; }
     n = nb;
     const int images = cfg.images;
@@ -689,9 +723,12 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     k_gather_pos<<<blocks(n), 256, 0, st>>>(n, perm.as<int32_t>(), dx, dy, dz, x.as<double>(), y.as<double>(), z.as<double>());
     CK(hipGetLastError());
     CK(scan_incl(temp, head.as<int32_t>(), rank.as<int32_t>(), n, st));
-    int32_t nlf = 0;
-    CK(read_back(rank.as<int32_t>() + (n - 1), &nlf, 1, st));
-    nleaf = nlf;
+    // one read-back for the leaf count (after lvl_nl[kD]) and the non-leaf cells per level
+    k_copy_last<<<1, 1, 0, st>>>(rank.as<int32_t>(), n - 1, lvl_nl.as<int32_t>() + kD);   // -> lvl_nl[kD + 1]
+    CK(hipGetLastError());
+    std::vector<int32_t> lvl_count(kD + 2, 0);
+    CK(read_back(lvl_nl.as<int32_t>(), lvl_count.data(), (size_t)kD + 2, st));
+    nleaf = lvl_count[kD + 1];
     CK(leaf_cell.ensure((size_t)nleaf * 4)); CK(leaf_depth.ensure((size_t)nleaf * 4));
     k_leaf_list<<<blocks(ntree), 256, 0, st>>>(ntree, cnum.as<int32_t>(), boff.as<int32_t>(), depth.as<int32_t>(),
                                                 rank.as<int32_t>(), leaf_cell.as<int32_t>(), leaf_depth.as<int32_t>());
@@ -708,8 +745,6 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     k_nonleaf_flag<<<blocks(ntree), 256, 0, st>>>(ntree, cnum.as<int32_t>(), flags.as<int32_t>());
     CK(hipGetLastError());
     CK(scan_incl(temp, flags.as<int32_t>(), nl_idx.as<int32_t>(), ntree, st));
-    std::vector<int32_t> lvl_count(maxd + 1, 0);
-    CK(read_back(lvl_nl.as<int32_t>(), lvl_count.data(), (size_t)maxd + 1, st));
     std::vector<int32_t> info(4 * (maxd + 1), 0);
     m2m_levels.clear(); l2l_levels.clear();
     max_level_cols = 1;
@@ -730,7 +765,7 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
         col += ncol;
     }
     for (int l = maxd - 1; l >= 0; l--) m2m_levels.push_back(l2l_levels[l]);
-    CK(upload(lvl_info, info, st));
+    CK(upload_staged(stage, lvl_info, info, st));
     CK(m2m_in.ensure(std::max<int64_t>(tab, 1) * 4)); CK(l2l_in.ensure(std::max<int64_t>(tab, 1) * 4));
     CK(l2l_out.ensure(std::max<int64_t>(tab, 1) * 4)); CK(m2m_out.ensure(std::max<int64_t>(col, 1) * 4));
     if (maxd > 0) {
@@ -753,7 +788,7 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
         std::vector<int32_t> ia, ib;
         for (auto& p : img_m2m) { ia.push_back(p[0]); ib.push_back(p[1]); }
         for (auto& p : img_l2l) { ia.push_back(p[0]); ib.push_back(p[1]); }
-        CK(upload(img_a, ia, st)); CK(upload(img_b, ib, st));
+        CK(upload_staged(stage, img_a, ia, st)); CK(upload_staged(stage, img_b, ib, st));
     }
     // --- 5. M2L tables (targets = sources = the non-leaf cells, counted per level above)
     nl = nls;
@@ -774,7 +809,7 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     if (nimg > 0) {   // image targets c(-1..-(images-1)): 26 self entries through the 1x27 lattice kernel
         std::vector<double> sc(nimg);
         for (int m = 1; m <= nimg; m++) sc[m - 1] = std::pow(-m >= -1 ? 2.0 : 3.0, -m + 1);   // kernel.cpp:1372
-        CK(upload(img_scale, sc, st));
+        CK(upload_staged(stage, img_scale, sc, st));
         CK(img_tar.ensure((size_t)nimg * 4));
         k_image_targets<<<blocks(nimg), 256, 0, st>>>(nl, nimg, ntree, ntar, img_scale.as<double>(),
                                                        nl_cells.as<int32_t>(), child_slots.as<int32_t>(),
@@ -798,15 +833,10 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
 // while the far-field stages of an evaluation run on the tree's stream.
 int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::string& err) {
     const int images = cfg.images;
-    // --- 6. P2P map by frontier traversal
-    std::vector<int32_t> init_tc, init_sc, init_tcp, init_scp, init_oi;
+    // --- 6. P2P map by frontier traversal, from (root, root) under each periodic offset
     const int top = images == 0 ? 0 : ntree;
-    for (int oi = 0; oi < 27; oi++) {
-        if (images == 0 && oi != 13) continue;
-        init_tc.push_back(0); init_sc.push_back(0); init_tcp.push_back(top); init_scp.push_back(top); init_oi.push_back(oi);
-    }
     int cur = 0;
-    int64_t nitems = (int64_t)init_tc.size();
+    int64_t nitems = images == 0 ? 1 : 27;
     auto ensure_frontier = [&](int b, int64_t cnt) -> hipError_t {
         if ((size_t)cnt <= f_cap[b]) return hipSuccess;
         size_t cap = std::max<size_t>(cnt, f_cap[b] * 2);
@@ -820,11 +850,9 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
         return hipSuccess;
     };
     CK(ensure_frontier(0, std::max<int64_t>(nitems, 1 << 16)));
-    CK(hipMemcpyAsync(f_tc[0].p, init_tc.data(), nitems * 4, hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(f_sc[0].p, init_sc.data(), nitems * 4, hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(f_tcp[0].p, init_tcp.data(), nitems * 4, hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(f_scp[0].p, init_scp.data(), nitems * 4, hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(f_oi[0].p, init_oi.data(), nitems * 4, hipMemcpyHostToDevice, st));
+    k_frontier_init<<<1, 32, 0, st>>>((int)nitems, top, f_tc[0].as<int32_t>(), f_sc[0].as<int32_t>(),
+                                      f_tcp[0].as<int32_t>(), f_scp[0].as<int32_t>(), f_oi[0].as<int32_t>());
+    CK(hipGetLastError());
     unsigned long long* d_lost = reinterpret_cast<unsigned long long*>(small.as<char>() + 16);
     n_rows = 0;
     const CellView cv{cx.as<double>(), cy.as<double>(), cz.as<double>(), cr.as<double>(), depth.as<int32_t>(),

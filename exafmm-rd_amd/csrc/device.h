@@ -108,7 +108,8 @@ hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, c
 // partial[5 * block + (0..4)] = sum q x, sum q y, sum q z, sum q |x|^2, sum q
 hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 5 * nblocks */, int nblocks, hipStream_t st);
 // dL/dq and dL/dx from w = dL/dphi, grad phi, and (psi, grad psi) = the evaluation with charges w
-hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const double* partial_w, int nblocks,
+// partial_w holds nblocks + 1 rows of kMoments; row nblocks receives the summed moments.
+hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, double* partial_w, int nblocks,
                                   double c, double out_scale, const double* qin, const double* win,
                                   const double* grad_phi, const double* psi, const double* grad_psi, double* dq,
                                   double* dx, hipStream_t st);
@@ -116,10 +117,11 @@ hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const
 struct NearDev {
     const double *p = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr;
 };
-// mode: FMM3DPP_DIPOLE_* (0 off, 1 reference, 2 infinite-lattice gauge).  With `near` set, phi =
+// mode: FMM3DPP_DIPOLE_* (0 off, 1 reference, 2 infinite-lattice gauge); partial holds nblocks + 1
+// rows of kMoments (launch_dipole_partial's and one for their sum).  With `near` set, phi =
 // p + near.p and grad = g + near.g, the same roundings as the P2P accumulating onto p and g.
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
-                           const double* gy, const double* gz, NearDev near, const double* partial, int nblocks,
+                           const double* gy, const double* gz, NearDev near, double* partial, int nblocks,
                            int mode, double cycle, double out_scale, double* phi, double* grad, hipStream_t st);
 
 // Batches: moments and the final correction per segment (system) of the tree-ordered bodies;

@@ -49,6 +49,16 @@ struct GpuStructure {
     DBuf key, key_s, idx0, perm1, tkey, tkey_s, ldep, head, rank, nchild, cstart, scan32, packed, scan64, lvl_nl,
         ekey, eval, ekey_s, eval_s, cnt64, flags, lvl_info, small, temp;
     DBuf f_tc[2], f_sc[2], f_tcp[2], f_scp[2], f_oi[2];
+    // pinned host staging for the small host-built tables of a build: their copies stay in stream
+    // order without a host wait; the region is reused from the next build on (builds end synced)
+    struct PinnedStage {
+        char* p = nullptr;
+        size_t cap = 0, off = 0;
+        PinnedStage() = default;
+        PinnedStage(const PinnedStage&) = delete;
+        PinnedStage& operator=(const PinnedStage&) = delete;
+        ~PinnedStage() { if (p) (void)hipHostFree(p); }
+    } stage;
     size_t f_cap[2] = {0, 0};
 
     // build = build_tree (keys, sort, levels, leaves, operator-apply and M2L tables) + build_lists (the

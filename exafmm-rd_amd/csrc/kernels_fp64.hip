@@ -143,13 +143,16 @@ __global__ __launch_bounds__(256) void k_dipole_partial(BodiesDev b, int n, doub
     if (tid < kMoments) partial[kMoments * blockIdx.x + tid] = s[tid][0];
 }
 
-__device__ void sum_moments(const double* __restrict__ partial, int nblocks, double* M) {
-    if (threadIdx.x < kMoments) {
-        double a = 0;
-        for (int k = 0; k < nblocks; k++) a += partial[kMoments * k + threadIdx.x];
-        M[threadIdx.x] = a;
-    }
-    __syncthreads();
+
+// Moments of each segment (a system; the whole body set for a plan) in a fixed order: the sum of the
+// segment's per-block partials, block by block.
+__global__ __launch_bounds__(64) void k_moments_seg(const double* __restrict__ partial, int nblocks,
+                                                    double* __restrict__ moments) {
+    const int seg = blockIdx.x, k = threadIdx.x;
+    if (k >= kMoments) return;
+    double a = 0;
+    for (int j = 0; j < nblocks; j++) a += partial[((size_t)seg * nblocks + j) * kMoments + k];
+    moments[seg * kMoments + k] = a;
 }
 
 // Dipole correction, 1/(4 pi) scaling (src/body.cpp:88-96) and the return to input order
@@ -159,11 +162,9 @@ __device__ void sum_moments(const double* __restrict__ partial, int nblocks, dou
 __global__ __launch_bounds__(256) void k_finalize(int n, const int32_t* __restrict__ perm, BodiesDev b,
                                                   const double* __restrict__ p, const double* __restrict__ gx,
                                                   const double* __restrict__ gy, const double* __restrict__ gz,
-                                                  NearDev near, const double* __restrict__ partial, int nblocks,
-                                                  int mode, double cycle, double out_scale, double* __restrict__ phi,
+                                                  NearDev near, const double* __restrict__ M, int mode,
+                                                  double cycle, double out_scale, double* __restrict__ phi,
                                                   double* __restrict__ grad) {
-    __shared__ double M[kMoments];
-    sum_moments(partial, nblocks, M);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double pv = p[i], g0 = gx[i], g1 = gy[i], g2 = gz[i];
@@ -200,15 +201,13 @@ __global__ __launch_bounds__(256) void k_finalize(int n, const int32_t* __restri
 // with W = sum w_i, c = 4 pi / 3V in the infinite-lattice gauge and c = 0 without a dipole term.
 // Tree order in, input order out; Wm holds the moments of w (k_dipole_partial of the w charges).
 __global__ __launch_bounds__(256) void k_adjoint_combine(int n, const int32_t* __restrict__ perm, BodiesDev b,
-                                                         const double* __restrict__ partial_w, int nblocks, double c,
+                                                         const double* __restrict__ Wm, double c,
                                                          double out_scale, const double* __restrict__ qin,
                                                          const double* __restrict__ win,
                                                          const double* __restrict__ grad_phi,
                                                          const double* __restrict__ psi,
                                                          const double* __restrict__ grad_psi, double* __restrict__ dq,
                                                          double* __restrict__ dx) {
-    __shared__ double Wm[kMoments];
-    sum_moments(partial_w, nblocks, Wm);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int o = perm[i];
@@ -247,14 +246,6 @@ __global__ __launch_bounds__(256) void k_dipole_seg(BodiesDev b, const int64_t* 
     if (tid < kMoments) partial[((size_t)seg * gridDim.x + blockIdx.x) * kMoments + tid] = sm[tid][0];
 }
 
-__global__ __launch_bounds__(64) void k_moments_seg(const double* __restrict__ partial, int nblocks,
-                                                    double* __restrict__ moments) {
-    const int seg = blockIdx.x, k = threadIdx.x;
-    if (k >= kMoments) return;
-    double a = 0;
-    for (int j = 0; j < nblocks; j++) a += partial[((size_t)seg * nblocks + j) * kMoments + k];
-    moments[seg * kMoments + k] = a;
-}
 
 __global__ __launch_bounds__(256) void k_finalize_seg(int64_t n, const int32_t* __restrict__ perm, BodiesDev b,
                                                       const double* __restrict__ p, const double* __restrict__ gx,
@@ -329,25 +320,35 @@ hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial, int nblock
 }
 
 hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double* p, const double* gx,
-                           const double* gy, const double* gz, NearDev near, const double* partial, int nblocks,
+                           const double* gy, const double* gz, NearDev near, double* partial, int nblocks,
                            int mode, double cycle, double out_scale, double* phi, double* grad, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (!perm || !p || !gx || !gy || !gz || !partial || !phi || !grad || !valid(b)) return hipErrorInvalidValue;
+    if (!perm || !p || !gx || !gy || !gz || !partial || !phi || !grad || !valid(b) || nblocks <= 0)
+        return hipErrorInvalidValue;
     if (near.p && (!near.gx || !near.gy || !near.gz)) return hipErrorInvalidValue;
-    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, near, partial, nblocks, mode, cycle,
-                                                out_scale, phi, grad);
+    double* M = partial + (size_t)nblocks * kMoments;   // row nblocks: the summed moments
+    k_moments_seg<<<1, 64, 0, st>>>(partial, nblocks, M);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, near, M, mode, cycle, out_scale, phi,
+                                                grad);
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, const double* partial_w, int nblocks,
+hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, double* partial_w, int nblocks,
                                   double c, double out_scale, const double* qin, const double* win,
                                   const double* grad_phi, const double* psi, const double* grad_psi, double* dq,
                                   double* dx, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (!perm || !partial_w || !qin || !win || !grad_phi || !psi || !grad_psi || !dq || !dx || !valid(b))
+    if (!perm || !partial_w || !qin || !win || !grad_phi || !psi || !grad_psi || !dq || !dx || !valid(b) ||
+        nblocks <= 0)
         return hipErrorInvalidValue;
-    k_adjoint_combine<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, partial_w, nblocks, c, out_scale, qin, win,
-                                                        grad_phi, psi, grad_psi, dq, dx);
+    double* Wm = partial_w + (size_t)nblocks * kMoments;   // row nblocks: the summed moments of w
+    k_moments_seg<<<1, 64, 0, st>>>(partial_w, nblocks, Wm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_adjoint_combine<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, Wm, c, out_scale, qin, win, grad_phi, psi,
+                                                        grad_psi, dq, dx);
     return hipGetLastError();
 }
 

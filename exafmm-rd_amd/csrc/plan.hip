@@ -163,7 +163,7 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
         HIPCK(plan->d_Qk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
         HIPCK(plan->d_Pk.ensure(F * std::max(g.ntar, 1) * 8 * 16));
     }
-    HIPCK(plan->d_partial.ensure(256 * 5 * 8));
+    HIPCK(plan->d_partial.ensure((256 + 1) * 5 * 8));   // 256 block partials and their sum
     HIPCK(plan->d_m2m_part.ensure((size_t)8 * std::max(1, g.max_level_cols) * Sn * 8));
     HIPCK(hipStreamSynchronize(s));
     plan->host_ms["tree_and_lists"] = g.ms_build;
