@@ -28,10 +28,11 @@ namespace {
 const char* kStageNames[] = {"P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize"};
 constexpr int kStages = 9;
 // Events: 0..kStages are the stage boundaries on the plan's stream (7: end of L2P, 8: near field
-// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream.  The P2P is forked at ev[4],
-// the end of the 8x8 Hadamard, so it overlaps the down transform, L2L and L2P rather than the
-// Hadamard (both would slow each other down by about the P2P's own time).
-constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvents = kStages + 3, kEvForkNear = 4;
+// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream.  The P2P is forked at ev[5],
+// the end of the M2L, so it overlaps L2L and L2P rather than the Hadamard or the down transform
+// (overlapping the Hadamard slowed both by about the P2P's own time; after the M2L the step is
+// 0.5% shorter than after the Hadamard).
+constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvents = kStages + 3, kEvForkNear = 5;
 }  // namespace
 
 struct fmm3dpp_plan {
@@ -184,7 +185,7 @@ int launch_near(fmm3dpp_plan* plan);
 
 // Far field of an evaluation (fmm.cpp:50-74 up to L2P): gather, P2M, M2M, M2L, L2L, L2P, on the
 // plan's stream.  Needs the tree (build_tree), not the near-field lists.  near_now: the lists are
-// built, so the P2P is queued on the side stream right after the 8x8 Hadamard.
+// built, so the P2P is queued on the side stream right after the M2L.
 int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     hipStream_t s = plan->stream;
     const Operators& o = plan->ops;
@@ -239,10 +240,6 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
                                plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
-        if (near_now) {
-            int rc = launch_near(plan);
-            if (rc) return rc;
-        }
         HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
                                 plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
         HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
@@ -253,10 +250,6 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
                                plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
         HIPCK(hipEventRecord(plan->ev[4], s));
-        if (near_now) {
-            int rc = launch_near(plan);
-            if (rc) return rc;
-        }
         HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
         if (nimg > 0) {
             HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
@@ -268,6 +261,10 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         }
     }
     HIPCK(hipEventRecord(plan->ev[5], s));
+    if (near_now) {
+        int rc = launch_near(plan);
+        if (rc) return rc;
+    }
     // L2L (fmm.cpp:195-229): image levels from the outermost image, then one launch per tree level
     const size_t nimgm2m = g.img_m2m.size();
     for (size_t i = 0; i < g.img_l2l.size(); i++) {
@@ -297,7 +294,7 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     return FMM3DPP_OK;
 }
 
-// Near field (P2P, fmm.cpp:259-294) on the side stream, from the end of the 8x8 Hadamard on
+// Near field (P2P, fmm.cpp:259-294) on the side stream, from the end of the M2L on
 // (ev[kEvForkNear], recorded after the charges are in tree order); stored apart from the far field,
 // which the finalize adds.  Needs the lists.
 int launch_near(fmm3dpp_plan* plan) {
