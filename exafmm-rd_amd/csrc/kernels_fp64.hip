@@ -35,6 +35,7 @@ __global__ void k_gather(const double* __restrict__ qin, const int32_t* __restri
 // prefetched into registers.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int kGemmSmallCols = 128;   // column counts up to this go to k_gemm_cols_small
 __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
     __shared__ double As[TK][TM + 4];   // operator rows r (components), k-major
     __shared__ double Bs[TK][TN + 4];   // gathered vectors v, k-major
@@ -106,6 +107,52 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
                 *y = g.accumulate ? *y + v : v;
             }
         }
+}
+
+// Few columns (the upper tree levels and the image levels): one wave per 16 x 16 output block,
+// operands read straight from L2 with eight K steps of loads in flight together, instead of one
+// LDS stage of 16 k behind a barrier per step (a 64-column level took ~22 us, all latency).  Every
+// output element sees the same MFMA sequence as in k_gemm_cols (K in steps of 4 from k = 0, the
+// same operand lanes; k_gemm_cols's zero-padded steps past K add exact zeros), so the results are
+// the same bits.
+__global__ __launch_bounds__(64) void k_gemm_cols_small(GemmCols g) {
+    const int lane = threadIdx.x, n = lane & 15, kq = lane >> 4;
+    const int rb = blockIdx.x, vb = blockIdx.y, z = blockIdx.z;   // 16 components x 16 vectors
+    const int S = g.S, K = g.S;
+    const double* A = g.A + (int64_t)z * g.a_zstride;
+    const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
+    const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
+    double* Y = g.Y + (int64_t)z * g.y_zstride;
+    const int r = rb * 16 + n, cv = vb * 16 + n;   // the component and the vector this lane feeds
+    const int src = cv < g.ncol ? in_idx[cv] : -1;
+    const double* arow = r < S ? A + (int64_t)r * K : nullptr;
+    const double* xcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
+    f64x4 acc = {0, 0, 0, 0};
+    const int nk4 = (K + 3) / 4;
+    constexpr int U = 8;
+    for (int k40 = 0; k40 < nk4; k40 += U) {
+        double xv[U], av[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = 4 * (k40 + u) + kq;
+            xv[u] = (xcol && k < K) ? xcol[k] : 0.0;
+            av[u] = (arow && k < K) ? arow[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k40 + u < nk4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[u], av[u], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = vb * 16 + kq + 4 * q;
+        if (c >= g.ncol || r >= S) continue;
+        const int oi = out_idx[c];
+        if (oi < 0) continue;
+        const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
+        const double v = sc * acc[q];
+        double* y = &Y[(int64_t)oi * S + r];
+        *y = g.accumulate ? *y + v : v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, const double* __restrict__ part,
@@ -299,6 +346,11 @@ hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
     if (g.ncol <= 0) return hipSuccess;
     if (!g.A || !g.X || !g.Y || !g.in_idx || !g.out_idx || g.S <= 0 || g.nz <= 0) return hipErrorInvalidValue;
+    if (g.ncol <= kGemmSmallCols) {   // same bits as the tiled kernel
+        dim3 grid((g.S + 15) / 16, (g.ncol + 15) / 16, g.nz);
+        k_gemm_cols_small<<<grid, 64, 0, st>>>(g);
+        return hipGetLastError();
+    }
     dim3 grid((g.S + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
     k_gemm_cols<<<grid, 256, 0, st>>>(g);
     return hipGetLastError();
