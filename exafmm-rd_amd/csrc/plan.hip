@@ -133,8 +133,10 @@ int ensure_ops(fmm3dpp_plan* plan) {
 
 // Device tree/lists plus the per-evaluate work buffers sized for them.
 // with_lists = false leaves the near-field lists to GpuStructure::build_lists (fmm3dpp_solve_device).
+// sync = false leaves the tree's last kernels queued on the plan's stream (solve_device queues the
+// far field behind them); every host-side count the evaluation needs is known on return either way.
 int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* dy, const double* dz,
-                 bool with_lists = true) {
+                 bool with_lists = true, bool sync = true) {
     hipStream_t s = plan->stream;
     int rc = with_lists ? plan->gs.build(plan->cfg, n, dx, dy, dz, s, plan->err)
                         : plan->gs.build_tree(plan->cfg, n, dx, dy, dz, s, plan->err);
@@ -166,7 +168,7 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
     }
     HIPCK(plan->d_partial.ensure((256 + 1) * 5 * 8));   // 256 block partials and their sum
     HIPCK(plan->d_m2m_part.ensure((size_t)8 * std::max(1, g.max_level_cols) * Sn * 8));
-    HIPCK(hipStreamSynchronize(s));
+    if (sync) HIPCK(hipStreamSynchronize(s));
     plan->host_ms["tree_and_lists"] = g.ms_build;
     return FMM3DPP_OK;
 }
@@ -485,7 +487,7 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
     int rc = ensure_ops(plan);
     if (rc) return rc;
     const double t0 = now_ms();
-    rc = build_device(plan, n, d_x, d_y, d_z, false);   // tree on the plan's stream, completed on return
+    rc = build_device(plan, n, d_x, d_y, d_z, false, false);   // tree on the plan's stream
     if (rc) return rc;
     plan->host_ms["tree"] = now_ms() - t0;
     rc = eval_far(plan, d_q, false);                    // queued on the plan's stream

@@ -231,7 +231,7 @@ double fmm3dpp_batch_timing_ms(const fmm3dpp_batch* batch, const char* name);
  * of the near field), evaluate (for solve_device: from the first far-field launch, so it includes
  * the list build that overlaps it), near_start / far_end (from the first far-field launch to the
  * start of the P2P / the end of L2P); host wall times: tree_and_lists (last
- * set_particles), tree (last solve_device, the tree build before the far field starts),
+ * set_particles), tree (last solve_device: host time of the tree build until the far field is queued),
  * precompute.  -1 if unknown. */
 double fmm3dpp_timing_ms(const fmm3dpp_plan* plan, const char* name);
 
