@@ -5,10 +5,11 @@
 //   Qk, Pk   [parent][f][8 children] complex T, i.e. one 16-real vector per (parent, f): the
 //            Hadamard reads and writes whole 128-byte (float64) or 64-byte (float32) vectors, and the
 //            up/down transforms stream each parent's spectrum contiguously.  The Hadamard runs one
-//            frequency per workgroup with consecutive frequencies on one XCD, so the vectors of one
-//            frequency are re-read from that XCD's L2.
+//            (frequency, chunk of targets) per workgroup with the chunks of one frequency on one
+//            XCD, and copies each chunk's distinct source vectors into LDS once.
 //   inv_lat  [k][j][i] -> surface point index s on the P^3 corner of the convolution grid, or -1
 //   gk8      [26 patterns][f][is * 8 + it] complex T
+//   gkA      [f][26 patterns][64 lanes] 4-vectors of T: gk8 as matrix-core A operands in lane order
 // Each kernel cites the reference loop it replaces (paths under 3dpp/).
 #include <hip/hip_runtime.h>
 
