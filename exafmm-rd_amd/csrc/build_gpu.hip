@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -590,6 +591,53 @@ hipError_t read_back(const T* d, T* h, size_t cnt, hipStream_t st) {
     return hipStreamSynchronize(st);
 }
 
+__global__ void k_publish(const uint32_t* __restrict__ src, int nwords, uint32_t* dst, uint32_t seq) {
+    const int i = threadIdx.x;
+    if (i < nwords) dst[i] = src[i];
+    __threadfence_system();
+    __syncthreads();
+    if (i == 0) {
+        dst[GpuStructure::Mailbox::kWords] = seq;
+        __threadfence_system();
+    }
+}
+
+// A few values the host needs before it can queue more work: published to mapped pinned memory by a
+// kernel queued behind everything before it on st, the host polling for the sequence number (no
+// copy engine and no stream synchronisation).  Falls back to read_back when the mailbox is missing
+// or too small; after 5 s without the sequence number the stream is synchronised and its error, if
+// any, returned.
+template <class T>
+hipError_t small_read(GpuStructure::Mailbox& mb, const T* d, T* h, size_t cnt, hipStream_t st) {
+    const size_t bytes = cnt * sizeof(T);
+    if (bytes % 4 != 0 || bytes / 4 > (size_t)GpuStructure::Mailbox::kWords) return read_back(d, h, cnt, st);
+    hipError_t e;
+    if (!mb.h) {
+        e = hipHostMalloc((void**)&mb.h, 64 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) {
+            mb.h = nullptr;
+            return e;
+        }
+        if ((e = hipHostGetDevicePointer((void**)&mb.d, mb.h, 0)) != hipSuccess) return e;
+        mb.h[GpuStructure::Mailbox::kWords] = mb.seq;
+    }
+    const uint32_t seq = ++mb.seq;
+    k_publish<<<1, 64, 0, st>>>(reinterpret_cast<const uint32_t*>(d), (int)(bytes / 4), mb.d, seq);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    volatile uint32_t* flag = mb.h + GpuStructure::Mailbox::kWords;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (*flag != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5)) {
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+            if (*flag != seq) return hipErrorUnknown;
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    std::memcpy(h, const_cast<const uint32_t*>(static_cast<volatile uint32_t*>(mb.h)), bytes);
+    return hipSuccess;
+}
+
 int bits_for(int64_t v) {
     int b = 1;
     while ((int64_t(1) << b) <= v) b++;
@@ -673,7 +721,7 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
             k_copy_last<<<1, 1, 0, st>>>(scan32.as<int32_t>(), ncl - 1, flag);
             CK(hipGetLastError());
             int32_t fl[2] = {0, 0};   // error flag, children total
-            CK(read_back(flag, fl, 2, st));
+            CK(small_read(mailbox, flag, fl, 2, st));
             hflag = fl[0];
             const int32_t total = fl[1];
             if (hflag & 1) { err = "non-finite body position"; return FMM3DPP_ERR_ARG; }
@@ -727,7 +775,7 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     k_copy_last<<<1, 1, 0, st>>>(rank.as<int32_t>(), n - 1, lvl_nl.as<int32_t>() + kD);   // -> lvl_nl[kD + 1]
     CK(hipGetLastError());
     std::vector<int32_t> lvl_count(kD + 2, 0);
-    CK(read_back(lvl_nl.as<int32_t>(), lvl_count.data(), (size_t)kD + 2, st));
+    CK(small_read(mailbox, lvl_nl.as<int32_t>(), lvl_count.data(), (size_t)kD + 2, st));
     nleaf = lvl_count[kD + 1];
     CK(leaf_cell.ensure((size_t)nleaf * 4)); CK(leaf_depth.ensure((size_t)nleaf * 4));
     k_leaf_list<<<blocks(ntree), 256, 0, st>>>(ntree, cnum.as<int32_t>(), boff.as<int32_t>(), depth.as<int32_t>(),
@@ -866,7 +914,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
         CK(hipGetLastError());
         CK(scan_incl(temp, packed.as<int64_t>(), scan64.as<int64_t>(), nitems, st));
         int64_t tot = 0;
-        CK(read_back(scan64.as<int64_t>() + (nitems - 1), &tot, 1, st));
+        CK(small_read(mailbox, scan64.as<int64_t>() + (nitems - 1), &tot, 1, st));
         const int64_t new_rows = tot >> 32, new_items = tot & 0xffffffffll;
         if ((size_t)(n_rows + new_rows) > rows_cap) {
             size_t cap = std::max<size_t>((size_t)(n_rows + new_rows), rows_cap * 2);
@@ -891,7 +939,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
         cur = nxt;
     }
     unsigned long long hl = 0;
-    CK(read_back(d_lost, &hl, 1, st));
+    CK(small_read(mailbox, d_lost, &hl, 1, st));
     lost = (int64_t)hl;
     // --- 7. rows -> per-leaf CSR
     CK(p2p_off.ensure((size_t)(nleaf + 1) * 8));
@@ -902,7 +950,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
                                                     cnt64.as<int64_t>());
         CK(hipGetLastError());
         CK(scan_incl(temp, cnt64.as<int64_t>(), scan64.as<int64_t>(), n_rows, st));
-        CK(read_back(scan64.as<int64_t>() + (n_rows - 1), &n_entries, 1, st));
+        CK(small_read(mailbox, scan64.as<int64_t>() + (n_rows - 1), &n_entries, 1, st));
         CK(ekey.ensure((size_t)n_entries * 4)); CK(eval.ensure((size_t)n_entries * 4));
         CK(ekey_s.ensure((size_t)n_entries * 4)); CK(p2p_ent.ensure((size_t)n_entries * 4));
         k_p2p_expand<<<blocks(n_rows), 256, 0, st>>>(n_rows, rows_tc.as<int32_t>(), rows_sc.as<int32_t>(),

@@ -59,6 +59,18 @@ struct GpuStructure {
         PinnedStage& operator=(const PinnedStage&) = delete;
         ~PinnedStage() { if (p) (void)hipHostFree(p); }
     } stage;
+    // mapped pinned words for the small counts a build reads back: a one-wave kernel queued behind the
+    // build's kernels copies them there and then writes a sequence number the host polls for
+    struct Mailbox {
+        uint32_t* h = nullptr;   // host view; word kWords is the sequence number
+        uint32_t* d = nullptr;   // device view of the same memory
+        uint32_t seq = 0;
+        static constexpr int kWords = 63;
+        Mailbox() = default;
+        Mailbox(const Mailbox&) = delete;
+        Mailbox& operator=(const Mailbox&) = delete;
+        ~Mailbox() { if (h) (void)hipHostFree(h); }
+    } mailbox;
     size_t f_cap[2] = {0, 0};
 
     // build = build_tree (keys, sort, levels, leaves, operator-apply and M2L tables) + build_lists (the
