@@ -379,9 +379,11 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     {
         GemmCols g1{Sn, nleaf, 1, 0, g.dop.o_ut.as<double>(), 0, g.pcleaf.as<double>(), iota, iota, 0,
                     g.pcleaf2.as<double>(), 0, nullptr};
+        g1.M = g.dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
         BHIP(launch_gemm_cols(g1, s));
         GemmCols g2{Sn, nleaf, 1, 0, g.dop.o_vs.as<double>(), 0, g.pcleaf2.as<double>(), iota, g.leaf_cell.as<int32_t>(), 0,
                     qeq, 0, g.leaf_depth.as<int32_t>()};
+        g2.K = g.dop.pinv_rank;
         BHIP(launch_gemm_cols(g2, s));
     }
     // M2M per level, then the image levels (level m of every system in one launch)
@@ -438,9 +440,11 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     {
         GemmCols g1{Sn, nleaf, 1, 0, g.dop.o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
                     g.pcleaf2.as<double>(), 0, nullptr};
+        g1.M = g.dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
         BHIP(launch_gemm_cols(g1, s));
         GemmCols g2{Sn, nleaf, 1, 0, g.dop.o_vsl.as<double>(), 0, g.pcleaf2.as<double>(), iota, iota, 0,
                     g.pcleaf.as<double>(), 0, g.leaf_depth.as<int32_t>()};
+        g2.K = g.dop.pinv_rank;
         BHIP(launch_gemm_cols(g2, s));
     }
     BHIP(launch_l2p_eval(Sn, g.dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.pcleaf.as<double>(),

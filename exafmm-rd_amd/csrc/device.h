@@ -45,6 +45,9 @@ struct GemmCols {
     double* Y;
     int64_t y_zstride;
     const int32_t* scale_depth;
+    // Rows of A used (M) and the inner dimension (K), 0 for S: a leading block of the S x S operator
+    // (row stride S), as the truncated pseudo-inverse applies use.  Vectors keep stride S.
+    int M = 0, K = 0;
 };
 
 hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int n, hipStream_t st);

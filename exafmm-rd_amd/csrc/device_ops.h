@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "dbuf.h"
@@ -20,6 +22,25 @@ struct DeviceOperators {
     DBuf o_usurf;                    // unit check/equivalent surface
     DBuf o_invlat;                   // P^3 grid corner (k * P + j) * P + i -> surface point, -1 inside
     DBuf o_tw;                       // FFT twiddles of length N = 2P - 1
+    // Rank at which the mixed path applies the P2M / L2P pseudo-inverses (0: full, S): the leading
+    // rows of UT and columns of VSinv whose singular values exceed 1e-7 of the largest, rounded up
+    // to a multiple of 16.  The dropped directions change the field outside the equivalent surface
+    // by ~1e-8 relative (P = 8, 10), three orders below the float32 M2L's own error.
+    int pinv_rank = 0;
+
+    // Column j of VSinv = V diag(1 / s) has norm 1 / s_j, with s in descending order.
+    static int truncated_rank(const std::vector<double>& vs, int S) {
+        std::vector<double> n2(S, 0.0);
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) n2[j] += vs[(size_t)i * S + j] * vs[(size_t)i * S + j];
+        double smax = 0;
+        for (int j = 0; j < S; j++)
+            if (n2[j] > 0) smax = std::max(smax, 1.0 / std::sqrt(n2[j]));
+        int r = 0;
+        for (int j = 0; j < S; j++)
+            if (n2[j] > 0 && 1.0 / std::sqrt(n2[j]) > 1e-7 * smax) r = j + 1;
+        return std::min(S, (r + 15) / 16 * 16);
+    }
 
     hipError_t upload(const Operators& o, bool mixed, hipStream_t s) {
         std::vector<double> l2l_all, m2m_all;
@@ -47,6 +68,7 @@ struct DeviceOperators {
         up(o_usurf, o.unit_surf);
         up(o_invlat, inv);
         up(o_tw, o.twiddle);
+        pinv_rank = mixed ? truncated_rank(o.VSinv, o.S) : 0;
         if (e == hipSuccess) e = o_gkA.ensure((size_t)o.F * 26 * 64 * 4 * (mixed ? sizeof(float) : sizeof(double)));
         if (e == hipSuccess) e = launch_hadamard_lanes(mixed, o.F, mixed ? o_gk8f.p : o_gk8.p, o_gkA.p, s);
         return e;

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
     __shared__ double Bs[TK][TN + 4];   // gathered vectors v, k-major
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row0 = blockIdx.x * TM, col0 = blockIdx.y * TN, z = blockIdx.z;
-    const int S = g.S, K = g.S;
+    const int S = g.S, M = g.M ? g.M : S, K = g.K ? g.K : S;
     const double* A = g.A + (int64_t)z * g.a_zstride;
     const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
     const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
     const int lr = tid >> 2, lk = (tid & 3) * 4;
     const int gr = row0 + lr, gc = col0 + lr;
     const int src = gc < g.ncol ? in_idx[gc] : -1;
-    const double* arow = gr < S ? A + (int64_t)gr * K : nullptr;
+    const double* arow = gr < M ? A + (int64_t)gr * S : nullptr;
     const double* bcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
     double ra[4], rb[4];
     auto fetch = [&](int k0) {
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int r = row0 + (rb0 + j) * 16 + n;
-                if (r >= S) continue;
+                if (r >= M) continue;
                 const double v = sc * acc[i][j][q];
                 double* y = &Y[(int64_t)oi * S + r];
                 *y = g.accumulate ? *y + v : v;
@@ -118,14 +118,14 @@ __global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
 __global__ __launch_bounds__(64) void k_gemm_cols_small(GemmCols g) {
     const int lane = threadIdx.x, n = lane & 15, kq = lane >> 4;
     const int rb = blockIdx.x, vb = blockIdx.y, z = blockIdx.z;   // 16 components x 16 vectors
-    const int S = g.S, K = g.S;
+    const int S = g.S, M = g.M ? g.M : S, K = g.K ? g.K : S;
     const double* A = g.A + (int64_t)z * g.a_zstride;
     const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
     const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
     double* Y = g.Y + (int64_t)z * g.y_zstride;
     const int r = rb * 16 + n, cv = vb * 16 + n;   // the component and the vector this lane feeds
     const int src = cv < g.ncol ? in_idx[cv] : -1;
-    const double* arow = r < S ? A + (int64_t)r * K : nullptr;
+    const double* arow = r < M ? A + (int64_t)r * S : nullptr;
     const double* xcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
     f64x4 acc = {0, 0, 0, 0};
     const int nk4 = (K + 3) / 4;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(64) void k_gemm_cols_small(GemmCols g) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int c = vb * 16 + kq + 4 * q;
-        if (c >= g.ncol || r >= S) continue;
+        if (c >= g.ncol || r >= M) continue;
         const int oi = out_idx[c];
         if (oi < 0) continue;
         const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
@@ -346,12 +346,14 @@ hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int
 hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
     if (g.ncol <= 0) return hipSuccess;
     if (!g.A || !g.X || !g.Y || !g.in_idx || !g.out_idx || g.S <= 0 || g.nz <= 0) return hipErrorInvalidValue;
+    if (g.M < 0 || g.M > g.S || g.K < 0 || g.K > g.S) return hipErrorInvalidValue;
+    const int M = g.M ? g.M : g.S;
     if (g.ncol <= kGemmSmallCols) {   // same bits as the tiled kernel
-        dim3 grid((g.S + 15) / 16, (g.ncol + 15) / 16, g.nz);
+        dim3 grid((M + 15) / 16, (g.ncol + 15) / 16, g.nz);
         k_gemm_cols_small<<<grid, 64, 0, st>>>(g);
         return hipGetLastError();
     }
-    dim3 grid((g.S + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
+    dim3 grid((M + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
     k_gemm_cols<<<grid, 256, 0, st>>>(g);
     return hipGetLastError();
 }

@@ -210,9 +210,11 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     {
         GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
                     plan->d_pcleaf2.as<double>(), 0, nullptr};
+        g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
         HIPCK(launch_gemm_cols(g1, s));
         GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota,
                     g.leaf_cell.as<int32_t>(), 0, qeq, 0, g.leaf_depth.as<int32_t>()};
+        g2.K = plan->dop.pinv_rank;
         HIPCK(launch_gemm_cols(g2, s));
     }
     HIPCK(hipEventRecord(plan->ev[1], s));
@@ -284,9 +286,11 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     {
         GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
                     plan->d_pcleaf2.as<double>(), 0, nullptr};
+        g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
         HIPCK(launch_gemm_cols(g1, s));
         GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
                     plan->d_pcleaf.as<double>(), 0, g.leaf_depth.as<int32_t>()};
+        g2.K = plan->dop.pinv_rank;
         HIPCK(launch_gemm_cols(g2, s));
     }
     HIPCK(launch_l2p_eval(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
