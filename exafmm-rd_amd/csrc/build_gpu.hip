@@ -656,7 +656,10 @@ int GpuStructure::build(const fmm3dpp_config& cfg, int64_t nb, const double* dx,
 int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double* dx, const double* dy,
                              const double* dz, hipStream_t st, std::string& err) {
     t_build = std::chrono::steady_clock::now();
-    stage.off = 0;   // the previous build ended with its stream synchronised
+    // The pinned staging region is reused from its start: the previous build's copies out of it
+    // were queued before its tree_done event (solve_device returns without waiting for them).
+    if (tree_done.recorded) CK(hipEventSynchronize(tree_done.ev));
+    stage.off = 0;
     if (nb < 1 || nb > (int64_t)INT32_MAX / 2) { err = "number of bodies must be in [1, 2^30)"; return FMM3DPP_ERR_ARG; }
     n = nb;
     const int images = cfg.images;
@@ -874,6 +877,9 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
     CK(had_loc.ensure(nch * 26 * kHadChunk * 4));
     CK(launch_hadamard_tables(ntar_h, ntar, slot_src_T.as<int32_t>(), had_uniq.as<int32_t>(), had_cnt.as<int32_t>(),
                               had_loc.as<int32_t>(), st));
+    if (!tree_done.ev) CK(hipEventCreateWithFlags(&tree_done.ev, hipEventDisableTiming));
+    CK(hipEventRecord(tree_done.ev, st));
+    tree_done.recorded = true;
     return FMM3DPP_OK;
 }
 
@@ -881,6 +887,8 @@ int GpuStructure::build_tree(const fmm3dpp_config& cfg, int64_t nb, const double
 // while the far-field stages of an evaluation run on the tree's stream.
 int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::string& err) {
     const int images = cfg.images;
+    if (!tree_done.recorded) { err = "build_lists needs a tree"; return FMM3DPP_ERR_STATE; }
+    CK(hipStreamWaitEvent(st, tree_done.ev, 0));   // the tree's last kernels may still be queued on its stream
     // --- 6. P2P map by frontier traversal, from (root, root) under each periodic offset
     const int top = images == 0 ? 0 : ntree;
     int cur = 0;
@@ -912,7 +920,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
                                                             f_oi[cur].as<int32_t>(), cv, cfg.cycle, packed.as<int64_t>(),
                                                             cnt64.as<int8_t>(), d_lost);
         CK(hipGetLastError());
-        CK(scan_incl(temp, packed.as<int64_t>(), scan64.as<int64_t>(), nitems, st));
+        CK(scan_incl(temp_lists, packed.as<int64_t>(), scan64.as<int64_t>(), nitems, st));
         int64_t tot = 0;
         CK(small_read(mailbox, scan64.as<int64_t>() + (nitems - 1), &tot, 1, st));
         const int64_t new_rows = tot >> 32, new_items = tot & 0xffffffffll;
@@ -949,7 +957,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
                                                     boff.as<int32_t>(), bnum.as<int32_t>(), rank.as<int32_t>(),
                                                     cnt64.as<int64_t>());
         CK(hipGetLastError());
-        CK(scan_incl(temp, cnt64.as<int64_t>(), scan64.as<int64_t>(), n_rows, st));
+        CK(scan_incl(temp_lists, cnt64.as<int64_t>(), scan64.as<int64_t>(), n_rows, st));
         CK(small_read(mailbox, scan64.as<int64_t>() + (n_rows - 1), &n_entries, 1, st));
         CK(ekey.ensure((size_t)n_entries * 4)); CK(eval.ensure((size_t)n_entries * 4));
         CK(ekey_s.ensure((size_t)n_entries * 4)); CK(p2p_ent.ensure((size_t)n_entries * 4));
@@ -958,7 +966,7 @@ int GpuStructure::build_lists(const fmm3dpp_config& cfg, hipStream_t st, std::st
                                                      rank.as<int32_t>(), leaf_cell.as<int32_t>(), cnt64.as<int64_t>(),
                                                      scan64.as<int64_t>(), ekey.as<int32_t>(), eval.as<int32_t>());
         CK(hipGetLastError());
-        CK(sort_pairs(temp, ekey.as<uint32_t>(), ekey_s.as<uint32_t>(), eval.as<int32_t>(), p2p_ent.as<int32_t>(),
+        CK(sort_pairs(temp_lists, ekey.as<uint32_t>(), ekey_s.as<uint32_t>(), eval.as<int32_t>(), p2p_ent.as<int32_t>(),
                       n_entries, bits_for(nleaf), st));
     } else {
         n_entries = 0;

@@ -48,6 +48,7 @@ struct GpuStructure {
     // temporaries
     DBuf key, key_s, idx0, perm1, tkey, tkey_s, ldep, head, rank, nchild, cstart, scan32, packed, scan64, lvl_nl,
         ekey, eval, ekey_s, eval_s, cnt64, flags, lvl_info, small, temp;
+    DBuf temp_lists;   // rocPRIM scratch of build_lists, apart from build_tree's `temp` (another stream)
     DBuf f_tc[2], f_sc[2], f_tcp[2], f_scp[2], f_oi[2];
     // pinned host staging for the small host-built tables of a build: their copies stay in stream
     // order without a host wait; the region is reused from the next build on (builds end synced)
@@ -72,6 +73,18 @@ struct GpuStructure {
         ~Mailbox() { if (h) (void)hipHostFree(h); }
     } mailbox;
     size_t f_cap[2] = {0, 0};
+    // Recorded on the build stream once build_tree has queued its last kernel and copy.  A list
+    // build on another stream waits for it (it reads the cells, leaves and ranks of the tree), and
+    // the next build_tree waits for it on the host before it reuses the pinned staging region,
+    // whose copies the previous build left in stream order.
+    struct TreeDone {
+        hipEvent_t ev = nullptr;
+        bool recorded = false;
+        TreeDone() = default;
+        TreeDone(const TreeDone&) = delete;
+        TreeDone& operator=(const TreeDone&) = delete;
+        ~TreeDone() { if (ev) (void)hipEventDestroy(ev); }
+    } tree_done;
 
     // build = build_tree (keys, sort, levels, leaves, operator-apply and M2L tables) + build_lists (the
     // P2P lists); build_lists reads only what build_tree wrote and may use another stream.

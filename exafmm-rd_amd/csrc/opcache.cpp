@@ -9,6 +9,7 @@
 // one that was saved.
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -66,14 +67,16 @@ struct Reader {
         get(&v, sizeof(T));
         return v;
     }
-    template <class T> bool arr(const std::string& name, std::vector<T>& a) {
+    // `expect`: the element count the plan's configuration implies; any other count in the file is
+    // refused before anything is allocated for it.
+    template <class T> bool arr(const std::string& name, std::vector<T>& a, uint64_t expect) {
         const uint32_t len = val<uint32_t>();
         if (!ok || len != name.size()) return ok = false;
         std::string got(len, '\0');
         get(&got[0], len);
         const uint8_t type = val<uint8_t>();
         const uint64_t n = val<uint64_t>();
-        if (!ok || got != name || type != (std::is_same<T, double>::value ? 0 : 1) || n > (uint64_t(1) << 34))
+        if (!ok || got != name || type != (std::is_same<T, double>::value ? 0 : 1) || n != expect)
             return ok = false;
         a.resize(n);
         get(a.data(), n * sizeof(T));
@@ -102,6 +105,21 @@ void each_array(Operators& o, F&& fn) {
     fn("TWIDDLE", o.twiddle);
 }
 constexpr uint32_t kArrays = 4 + 8 + 27 + 8 + 1 + 2 + 3 + 3;
+
+// Element count of each array for (P, images), in the layouts of build_operators.
+uint64_t expected_size(const std::string& name, int P, int images) {
+    const uint64_t S = (uint64_t)surface_count(P), N = 2 * (uint64_t)P - 1, F = N * N * (N / 2 + 1);
+    const uint64_t SS = S * S, SSi = images > 0 ? SS : 0;
+    if (name == "GK8") return 26 * F * 128;
+    if (name == "GK27") return images >= 2 ? 26 * F * 56 : 0;
+    if (name == "GK27_SUM") return 26 * F * 2;
+    if (name == "M2M_CAT") return 8 * SS;
+    if (name == "M2M_IMG_SUM") return SS;
+    if (name == "UNIT_SURF" || name == "LATTICE") return 3 * S;
+    if (name == "TWIDDLE") return 2 * N;
+    if (name == "L2L_IMG" || name.rfind("M2M_IMG", 0) == 0) return SSi;
+    return SS;   // UT, VSinv, UT_l2p, VSinv_l2p, M2M0..7, L2L0..7
+}
 
 }  // namespace
 
@@ -139,7 +157,11 @@ int load_operators(Operators& ops, int P, int images, double cycle, const std::s
         return fail("operators are for P=" + std::to_string(fP) + " images=" + std::to_string(fimages) +
                     ", not for this plan's configuration");
     Operators o;
-    each_array(o, [&](const std::string& name, auto& a) { r.arr(name, a); });
+    try {
+        each_array(o, [&](const std::string& name, auto& a) { r.arr(name, a, expected_size(name, P, images)); });
+    } catch (const std::bad_alloc&) {
+        return fail("not enough host memory for the operators");
+    }
     const uint64_t want = r.hash.h;
     uint64_t got = 0;
     const bool tail = r.ok && fread(&got, 1, 8, f) == 8;
