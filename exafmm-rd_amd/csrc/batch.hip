@@ -50,6 +50,7 @@ struct Group {
     // merged device tables
     DBuf x, y, z, perm, cx, cy, cz, cr, depth, boff, bnum, leaf_cell, leaf_depth, iota, p2p_off, p2p_ent, m2m_in,
         m2m_out, l2l_in, l2l_out, img_a, img_b, child_slots, tar_scale, slot_src_T, img_src_T, img_tar, seg_off;
+    DBuf roots;   // merged index of every system's root cell
     DBuf had_uniq, had_cnt, had_loc;   // Hadamard staging tables of the merged targets
     HadTables had() const { return {had_uniq.as<int32_t>(), had_cnt.as<int32_t>(), had_loc.as<int32_t>()}; }
     DeviceOperators dop;
@@ -243,6 +244,7 @@ int build_group(fmm3dpp_batch* b, Group& g) {
     std::vector<std::vector<std::array<int32_t, 2>>> img_m2m_lv(std::max(images, 1)), img_l2l_lv(std::max(images, 1));
     std::vector<int32_t> img_tar;
     std::vector<int64_t> seg(1, 0);
+    std::vector<int32_t> roots;
     int64_t bo = 0, eo = 0;
     int co = 0, lo = 0, tl = 0, ti = nl;
     for (int k : g.sys) {
@@ -301,6 +303,7 @@ int build_group(fmm3dpp_batch* b, Group& g) {
         // image levels: (parent, child) cell pairs, level i of every system in one launch
         for (size_t i = 0; i < G.img_m2m.size(); i++) img_m2m_lv[i].push_back({G.img_m2m[i][0] + co, G.img_m2m[i][1] + co});
         for (size_t i = 0; i < G.img_l2l.size(); i++) img_l2l_lv[i].push_back({G.img_l2l[i][0] + co, G.img_l2l[i][1] + co});
+        roots.push_back(co);
         bo += G.n; eo += G.n_entries; co += G.ncells; lo += G.nleaf; tl += G.nl; ti += G.nimg;
         seg.push_back(bo);
     }
@@ -324,6 +327,7 @@ int build_group(fmm3dpp_batch* b, Group& g) {
     BHIP(upload(g.img_b, ib, s));
     BHIP(upload(g.img_tar, img_tar, s));
     BHIP(upload(g.seg_off, seg, s));
+    BHIP(upload(g.roots, roots, s));
     {   // Hadamard staging tables over the merged targets (tree parents only in mixed precision)
         const int ntar_h = b->mixed() ? nl : ntar;
         const size_t nch = (size_t)std::max(had_chunks(ntar_h), 1);
@@ -393,10 +397,14 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
         BHIP(launch_gemm_cols(gm, s));
         BHIP(launch_sum_partials(Sn, lv.ncol, 8, g.m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off, qeq, s));
     }
+    // infinite-lattice gauge: every system's root and image cells carry the zero monopole of a neutral box
+    const bool neutral = b->cfg[0].dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
+    if (neutral) BHIP(launch_neutralize(qeq, Sn, g.roots.as<int32_t>(), 0, (int)g.sys.size(), s));
     for (const auto& lv : g.img_m2m_lv) {
         GemmCols gi{Sn, (int)lv[1], 1, 0, g.dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + lv[0],
                     g.img_a.as<int32_t>() + lv[0], 0, qeq, 0, nullptr};
         BHIP(launch_gemm_cols(gi, s));
+        if (neutral) BHIP(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + lv[0], 0, (int)lv[1], s));
     }
     // M2L: targets = sources = all systems' non-leaf cells [0, nl), then all image cells [nl, ntar)
     const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;

@@ -166,6 +166,24 @@ __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, c
     Y[(int64_t)out[c] * S + r] = a;
 }
 
+// Infinite-lattice gauge (no counterpart in the reference): the equivalent density of a neutral
+// box's root carries a monopole of ~1e-10 of its size (the rounding the pseudo-inverse leaves); each
+// image level copies it 27 times and the 1x27 lattice M2L turns it into a potential offset that
+// grows 9x per image level.  Each listed cell's density is shifted uniformly so that its sum, the
+// cell's monopole, is zero.  One wave per cell, cell = cells[block] (or first + block without a
+// list); lane-strided partial sums, then an xor butterfly, after which every lane holds the same sum.
+__global__ __launch_bounds__(64) void k_neutralize(double* __restrict__ qeq, int S, const int32_t* __restrict__ cells,
+                                                   int first) {
+    const int c = cells ? cells[blockIdx.x] : first + (int)blockIdx.x;
+    if (c < 0) return;   // uniform over the wave
+    double* v = qeq + (int64_t)c * S;
+    double a = 0;
+    for (int j = threadIdx.x; j < S; j += 64) a += v[j];
+    for (int m = 32; m > 0; m >>= 1) a += __shfl_xor(a, m);
+    const double shift = -a / S;
+    for (int j = threadIdx.x; j < S; j += 64) v[j] += shift;
+}
+
 // Dipole moment sum_i q_i x_i (src/kernel.cpp:1587-1591), sum_i q_i |x_i|^2 and sum_i q_i as
 // per-block partial sums [block][kMoments].
 __global__ __launch_bounds__(256) void k_dipole_partial(BodiesDev b, int n, double* __restrict__ partial) {
@@ -364,6 +382,13 @@ hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, cons
     if (!part || !out || !Y || S <= 0) return hipErrorInvalidValue;
     int64_t tot = (int64_t)ncol * S;
     k_sum_partials<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, ncol, nz, part, out, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (!qeq || S <= 0 || (!cells && first < 0)) return hipErrorInvalidValue;
+    k_neutralize<<<count, 64, 0, st>>>(qeq, S, cells, first);
     return hipGetLastError();
 }
 

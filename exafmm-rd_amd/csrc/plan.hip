@@ -226,10 +226,14 @@ int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
         HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
                                   qeq, s));
     }
+    // infinite-lattice gauge: the root and every image cell carry the zero monopole of a neutral box
+    const bool neutral = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
+    if (neutral) HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, s));
     for (size_t i = 0; i < g.img_m2m.size(); i++) {
         GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
                     g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
         HIPCK(launch_gemm_cols(gi, s));
+        if (neutral) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, s));
     }
     HIPCK(hipEventRecord(plan->ev[2], s));
     // M2L (fmm.cpp:152-193, kernel.cpp:1140-1397)

@@ -684,8 +684,13 @@ static void stage_p2m(Plan& pl, const std::vector<int>& leaves, const double* q)
     }
 }
 
-// fmm.cpp:111-150 with kernel.cpp:395-404 and 434-442
-static void stage_m2m(Plan& pl, int mind, int maxd) {
+// fmm.cpp:111-150 with kernel.cpp:395-404 and 434-442.
+// neutral = true in the infinite-lattice gauge (dipole mode 2, no counterpart in the reference): the
+// root's and every image cell's equivalent density are shifted uniformly so that their sum (the
+// cell's monopole) is zero, the charge of a neutral box.  The pseudo-inverse leaves a monopole of
+// ~1e-10 of the density's size at the root; each image level copies it 27 times and the 1x27
+// lattice M2L turns it into a potential offset growing 9x per image level.
+static void stage_m2m(Plan& pl, int mind, int maxd, bool neutral = false) {
     int S = pl.S;
     for (int d = maxd; d > mind; d--) {
         std::vector<int> cs;
@@ -707,6 +712,12 @@ static void stage_m2m(Plan& pl, int mind, int maxd) {
                     gemv(pl.m2m_img[o].data(), qc, tmp.data(), S, S, 1.0);
                     for (int s = 0; s < S; s++) qp[s] += tmp[s];
                 }
+            }
+            if (neutral && d <= 0) {
+                double sum = 0;
+                for (int s = 0; s < S; s++) sum += qp[s];
+                const double shift = -sum / S;
+                for (int s = 0; s < S; s++) qp[s] += shift;
             }
         }
     }
@@ -1028,7 +1039,7 @@ int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* 
     double t0 = now();
     stage_p2m(*pl, leaves, q.data());
     double t1 = now();
-    stage_m2m(*pl, mind, maxd);
+    stage_m2m(*pl, mind, maxd, pl->dipole == 2);
     double t2 = now();
     stage_m2l(*pl);
     double t3 = now();
@@ -1038,7 +1049,18 @@ int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* 
     double t5 = now();
     stage_p2p(*pl, q.data(), p.data(), gx.data(), gy.data(), gz.data());
     double t6 = now();
-    if (pl->dipole) {   // kernel.cpp:1580-1596 (reference, energy-only potential term)
+    if (pl->dipole == 2) {   // infinite-lattice gauge (no counterpart in the reference)
+        const double coef = 4 * M_PI / (3 * pl->cycle * pl->cycle * pl->cycle);
+        double M[4] = {0, 0, 0, 0};
+        for (int i = 0; i < n; i++) {
+            M[0] += pl->bx[i] * q[i]; M[1] += pl->by[i] * q[i]; M[2] += pl->bz[i] * q[i];
+            M[3] += (pl->bx[i] * pl->bx[i] + pl->by[i] * pl->by[i] + pl->bz[i] * pl->bz[i]) * q[i];
+        }
+        for (int i = 0; i < n; i++) {
+            p[i] -= coef * (M[0] * pl->bx[i] + M[1] * pl->by[i] + M[2] * pl->bz[i]) - 0.5 * coef * M[3];
+            gx[i] -= coef * M[0]; gy[i] -= coef * M[1]; gz[i] -= coef * M[2];
+        }
+    } else if (pl->dipole) {   // kernel.cpp:1580-1596 (reference, energy-only potential term)
         double coef = 4 * M_PI / (3 * pl->cycle * pl->cycle * pl->cycle);
         double M[3] = {0, 0, 0};
         for (int i = 0; i < n; i++) { M[0] += pl->bx[i] * q[i]; M[1] += pl->by[i] * q[i]; M[2] += pl->bz[i] * q[i]; }

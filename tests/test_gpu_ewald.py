@@ -69,41 +69,64 @@ def test_ewald_alpha_independent(fmm):
     assert _rel(p1, p3) <= 1e-13 and _rel(g1, g3) <= 1e-13
 
 
-def test_physical_gauge_is_analytic_correction(fmm):
-    """DIPOLE_PHYSICAL = raw lattice sum (DIPOLE_OFF) - (4 pi/3V) M.x + (2 pi/3V) sum q |x|^2."""
+def _share_operators(plan, orc, images):
+    """The plan's operators in the oracle, in the structure the device applies them (the 27 image
+    M2M matrices and the 27 lattice kernels pre-summed)."""
+    import pyfmm3dpp
+    for name in pyfmm3dpp.OPERATOR_NAMES:
+        orc.set_operator(name, plan.get_operator(name))
+    S = plan.counts()["S"]
+    orc.set_operator("L2L_IMG", plan.get_operator("L2L_IMG"))
+    orc.set_operator("M2M_IMG0", plan.get_operator("M2M_IMG_SUM"))
+    for k in range(1, 27):
+        orc.set_operator(f"M2M_IMG{k}", np.zeros(S * S))
+    if images >= 2:
+        g = np.zeros_like(plan.get_operator("GK27")).reshape(26, -1, 28, 2)
+        g[:, :, 0, :] = plan.get_operator("GK27_SUM").reshape(26, -1, 2)
+        orc.set_operator("GK27", g)
+
+
+@pytest.mark.parametrize("images", [2, 6])
+def test_physical_gauge_matches_oracle(fmm, oracle_mod, images):
+    """FMM3DPP_DIPOLE_PHYSICAL against the oracle's infinite-lattice mode (zero monopole of the root
+    and image cells' equivalent densities, surface term of the cubic lattice sum) with shared
+    operators: float64, 1e-12 like every other parity case."""
     x, y, z, q = _uniform(8192, 21)
-    out = {}
-    for mode in (fmm.DIPOLE_OFF, fmm.DIPOLE_PHYSICAL):
-        plan = fmm.LaplaceFMM(6, 64, 2, 1.0, dipole_mode=mode, device=0)
-        plan.precompute()
-        out[mode] = plan.solve(x, y, z, q)
-    X = np.stack([x, y, z], 1)
-    M = (q[:, None] * X).sum(0)
-    c = 4 * np.pi / 3
-    p = out[fmm.DIPOLE_OFF][0] - c * (X @ M) + 0.5 * c * np.sum(q * (X * X).sum(1))
-    g = out[fmm.DIPOLE_OFF][1] - c * M
-    assert _rel(out[fmm.DIPOLE_PHYSICAL][0], p) <= 1e-13 and _rel(out[fmm.DIPOLE_PHYSICAL][1], g) <= 1e-13
+    plan = fmm.LaplaceFMM(8, 64, images, 1.0, dipole_mode=fmm.DIPOLE_PHYSICAL, device=0)
+    plan.precompute()
+    p, g = plan.solve(x, y, z, q)
+    orc = oracle_mod.OracleFMM(8, 64, images, 1.0, dipole=2, threads=16)
+    orc.precompute()
+    _share_operators(plan, orc, images)
+    orc.set_particles(x, y, z)
+    pr, gr = orc.evaluate(q)
+    print(f"images {images}: phi {_rel(p, pr):.2e} grad {_rel(g, gr):.2e}", flush=True)
+    assert _rel(p, pr) <= 1e-12 and _rel(g, gr) <= 1e-12
 
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_fmm_physical_gauge_vs_ewald(fmm, prec):
     """The lattice FMM in the infinite-lattice gauge converges to the Ewald sum as images grow (the
-    remainder of a (3^i)^3 cube decays like 9^-i); float64 and mixed precision alike.  A periodic
-    potential is defined up to an additive constant, which leaves forces and the energy of a neutral
-    system unchanged: the FMM carries a constant offset of about 1e-5 of the potential's norm (the
-    image levels' approximation error on the constant part of the lattice sum), so the potential is
-    compared after removing the mean difference over the targets; the raw figure is printed."""
+    remainder of a (3^i)^3 cube decays ~9x per level), float64 and mixed precision alike.  The raw
+    potential is compared (no constant removed): the zero monopole of the root and image cells keeps
+    the lattice free of the offset a rounding-level monopole would grow by 9x per image level.
+    Bounds: images=3 is the configuration's own lattice truncation (2.8e-4 measured on the CPU for
+    the potential, 1.4e-5 for the gradient); from 5 levels on both are below 1e-5 and falling."""
     x, y, z, q = _uniform(16384, 33)
     tar = np.arange(0, 16384, 4, dtype=np.int32)
     pe, ge = fmm.ewald(x, y, z, q, 1.0, 23, alpha=12.0, cutoff=0.5, targets=tar)
     errs = {}
-    for images in (3, 5):
+    for images in (3, 5, 6):
         plan = fmm.LaplaceFMM(8, 64, images, 1.0, dipole_mode=fmm.DIPOLE_PHYSICAL, precision=prec, device=0)
         plan.precompute()
         p, g = plan.solve(x, y, z, q)
         d = p[tar] - pe
-        errs[images] = (_rel(p[tar] - np.mean(d), pe), _rel(g[tar], ge))
-        print(f"precision {prec} images {images}: phi {errs[images][0]:.2e} (raw {_rel(p[tar], pe):.2e}, "
-              f"mean offset {np.mean(d):.3e}) grad {errs[images][1]:.2e}", flush=True)
-    assert errs[3][0] <= 1e-3 and errs[3][1] <= 2e-4
+        errs[images] = (_rel(p[tar], pe), _rel(g[tar], ge))
+        print(f"precision {prec} images {images}: raw phi {errs[images][0]:.2e} (mean offset {np.mean(d):.3e}, "
+              f"with it removed {_rel(p[tar] - np.mean(d), pe):.2e}) grad {errs[images][1]:.2e}", flush=True)
+    assert errs[3][0] <= 1e-3 and errs[3][1] <= 1e-4
     assert errs[5][0] <= 1e-5 and errs[5][1] <= 1e-5
+    assert errs[6][0] <= 1e-5 and errs[6][1] <= 1e-5
+    assert errs[5][0] < errs[3][0] / 10
+    if prec == 0:   # float64: the lattice remainder still dominates at 5 levels
+        assert errs[6][0] < errs[5][0]
