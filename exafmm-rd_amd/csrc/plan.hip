@@ -15,77 +15,9 @@
 #include <string>
 #include <vector>
 
-#include "dbuf.h"
-#include "device.h"
-#include "device_ops.h"
-#include "fmm3dpp.h"
-#include "gpu_structure.h"
-#include "structure.h"
+#include "plan.h"
 
 using namespace fmm3dpp;
-
-namespace {
-const char* kStageNames[] = {"P2M", "M2M", "M2L_up", "M2L_hadamard", "M2L_down", "L2L", "L2P", "P2P", "finalize"};
-constexpr int kStages = 9;
-// Events: 0..kStages are the stage boundaries on the plan's stream (7: end of L2P, 8: near field
-// joined); kEvNear0 / kEvNear1 bracket the P2P on the side stream.  The P2P is forked at ev[5],
-// the end of the M2L, so it overlaps L2L and L2P rather than the Hadamard or the down transform
-// (overlapping the Hadamard slowed both by about the P2P's own time; after the M2L the step is
-// 0.5% shorter than after the Hadamard).
-constexpr int kEvNear0 = kStages + 1, kEvNear1 = kStages + 2, kEvents = kStages + 3, kEvForkNear = 5;
-}  // namespace
-
-struct fmm3dpp_plan {
-    fmm3dpp_config cfg{};
-    int device = FMM3DPP_DEVICE_NONE;
-    std::string err;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    bool ops_ready = false, ops_uploaded = false, particles = false, host_struct_valid = false;
-    Operators ops;
-    Structure st;              // host tree/lists (structure-only plans; export cache for device plans)
-    GpuStructure gs;           // device tree/lists (device plans)
-    std::map<std::string, double> host_ms;
-    hipEvent_t ev[kEvents] = {};
-    // Near field (lists, P2P) beside the far field: created at the first evaluation that uses it, so
-    // that a process whose plans keep the near field on their own streams holds one stream per plan
-    // (HIP spreads streams over a few hardware queues; fewer streams, more plans on distinct queues).
-    hipStream_t side_stream = nullptr;
-    bool near_side = true;   // near field on side_stream (else on the plan's stream)
-    hipError_t ensure_near_stream() {
-        if (!near_side || side_stream) return hipSuccess;
-        return hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking);
-    }
-    hipStream_t near_stream() const { return near_side ? side_stream : stream; }
-    bool ev_recorded = false;
-    int had_cap = kHadCap;   // distinct sources a Hadamard chunk may stage through LDS (fmm3dpp_set_hadamard_staging)
-
-    DBuf d_inx, d_iny, d_inz, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
-    DBuf d_pn, d_gxn, d_gyn, d_gzn;   // near field, written by the P2P on the side stream
-    DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
-    DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
-    DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
-    DeviceOperators dop;
-
-    bool mixed() const { return cfg.precision == FMM3DPP_PREC_MIXED; }
-
-    CellsDev cells_dev() const {
-        return {gs.cx.as<double>(), gs.cy.as<double>(), gs.cz.as<double>(), gs.cr.as<double>(), gs.depth.as<int32_t>(),
-                gs.boff.as<int32_t>(), gs.bnum.as<int32_t>()};
-    }
-    BodiesDev bodies_dev() const { return {gs.x.as<double>(), gs.y.as<double>(), gs.z.as<double>(), d_q.as<double>()}; }
-};
-
-#define FAIL(code, msg)          \
-    do {                         \
-        plan->err = (msg);       \
-        return (code);           \
-    } while (0)
-#define HIPCK(call)                                                                              \
-    do {                                                                                         \
-        hipError_t e_ = (call);                                                                  \
-        if (e_ != hipSuccess) FAIL(FMM3DPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 namespace fmm3dpp {
 
@@ -118,6 +50,10 @@ int upload_operators(fmm3dpp_plan* plan) {
     return FMM3DPP_OK;
 }
 
+}  // namespace
+
+namespace fmm3dpp {
+
 int ensure_ops(fmm3dpp_plan* plan) {
     if (!plan->ops_ready) {
         double t0 = now_ms();
@@ -130,6 +66,153 @@ int ensure_ops(fmm3dpp_plan* plan) {
     if (plan->device >= 0 && !plan->ops_uploaded) return upload_operators(plan);
     return FMM3DPP_OK;
 }
+
+int ensure_host_structure(fmm3dpp_plan* plan) {
+    if (plan->host_struct_valid) return FMM3DPP_OK;
+    if (plan->device < 0 || !plan->particles) FAIL(FMM3DPP_ERR_STATE, "no particles");
+    HIPCK(hipSetDevice(plan->device));
+    int rc = plan->gs.export_structure(plan->cfg, plan->st, plan->stream, plan->err);
+    if (rc) return rc;
+    plan->host_struct_valid = true;
+    return FMM3DPP_OK;
+}
+
+// P2M (fmm.cpp:83-109, kernel.cpp:332-371): check potentials of the leaves' bodies, then
+// q_equiv = 0.5^depth * VSinv * (UT * p_check), UT and VSinv at the truncated rank on the mixed path.
+int stage_p2m_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t* leaf_depth, int nleaf) {
+    hipStream_t s = plan->stream;
+    const int Sn = plan->ops.S;
+    const int32_t* iota = plan->gs.iota.as<int32_t>();
+    HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), leaf_cell, nleaf, plan->cells_dev(), plan->bodies_dev(),
+                           plan->d_pcleaf.as<double>(), s));
+    GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
+                plan->d_pcleaf2.as<double>(), 0, nullptr};
+    g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
+    HIPCK(launch_gemm_cols(g1, s));
+    GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, leaf_cell, 0,
+                plan->d_qeq.as<double>(), 0, leaf_depth};
+    g2.K = plan->dop.pinv_rank;
+    HIPCK(launch_gemm_cols(g2, s));
+    return FMM3DPP_OK;
+}
+
+// M2M (fmm.cpp:111-150, kernel.cpp:395-404, 434-442): per tree level from the deepest, one GEMM per
+// child octant into partials and the child sum; then the image levels.  In the infinite-lattice
+// gauge the root and every image cell get the zero monopole of a neutral box.
+int stage_m2m_all(fmm3dpp_plan* plan) {
+    hipStream_t s = plan->stream;
+    const GpuStructure& g = plan->gs;
+    const int Sn = plan->ops.S;
+    const size_t SS = (size_t)Sn * Sn;
+    double* qeq = plan->d_qeq.as<double>();
+    const int32_t* iota = g.iota.as<int32_t>();
+    for (const auto& lv : g.m2m_levels) {
+        GemmCols gm{Sn, lv.ncol, 8, 0, plan->dop.o_m2m_all.as<double>(), (int64_t)SS, qeq,
+                    g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol, plan->d_m2m_part.as<double>(), 0, nullptr};
+        HIPCK(launch_gemm_cols(gm, s));
+        HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
+                                  qeq, s));
+    }
+    const bool neutral = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
+    if (neutral) HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, s));
+    for (size_t i = 0; i < g.img_m2m.size(); i++) {
+        GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
+                    g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
+        HIPCK(launch_gemm_cols(gi, s));
+        if (neutral) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, s));
+    }
+    return FMM3DPP_OK;
+}
+
+// M2L (fmm.cpp:152-193, kernel.cpp:1140-1397): up transform, 8x8 Hadamard, image lattice 1x27,
+// down transform accumulating into p_check.  Targets = sources = the tree's non-leaf cells [0, nl),
+// then the image cells [nl, ntar).  Records ev[3] after the up transform and ev[4] after the 8x8
+// Hadamard (stage timers).
+int stage_m2l_all(fmm3dpp_plan* plan) {
+    hipStream_t s = plan->stream;
+    const Operators& o = plan->ops;
+    const GpuStructure& g = plan->gs;
+    const int Sn = o.S;
+    double* qeq = plan->d_qeq.as<double>();
+    double* pck = plan->d_pck.as<double>();
+    const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;
+    const int32_t* slots = g.child_slots.as<int32_t>();
+    const int32_t* inv = plan->dop.o_invlat.as<int32_t>();
+    const double* tw = plan->dop.o_tw.as<double>();
+    if (!plan->mixed()) {
+        HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
+        HIPCK(hipEventRecord(plan->ev[3], s));
+        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
+                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
+        HIPCK(hipEventRecord(plan->ev[4], s));
+        HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
+                                plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
+        HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
+    } else {
+        // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
+        HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
+        HIPCK(hipEventRecord(plan->ev[3], s));
+        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
+                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
+        HIPCK(hipEventRecord(plan->ev[4], s));
+        HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
+        if (nimg > 0) {
+            HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
+            HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, nl,
+                                    plan->dop.o_gk27.as<double>(), plan->d_Qk_img.as<double>(),
+                                    plan->d_Pk_img.as<double>(), s));
+            HIPCK(launch_m2l_down(o.P, false, Sn, nimg, slots + (size_t)nl * 8, g.tar_scale.as<double>() + nl, inv,
+                                  tw, plan->d_Pk_img.p, pck, s));
+        }
+    }
+    return FMM3DPP_OK;
+}
+
+// L2L (fmm.cpp:195-229, kernel.cpp:1419-1435): image levels from the outermost image, then one
+// launch per tree level, each child's p_check += L2L[octant] * its parent's.
+int stage_l2l_all(fmm3dpp_plan* plan) {
+    hipStream_t s = plan->stream;
+    const GpuStructure& g = plan->gs;
+    const int Sn = plan->ops.S;
+    const size_t SS = (size_t)Sn * Sn;
+    double* pck = plan->d_pck.as<double>();
+    const size_t nimgm2m = g.img_m2m.size();
+    for (size_t i = 0; i < g.img_l2l.size(); i++) {
+        GemmCols gi{Sn, 1, 1, 1, plan->dop.o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + nimgm2m + i,
+                    g.img_b.as<int32_t>() + nimgm2m + i, 0, pck, 0, nullptr};
+        HIPCK(launch_gemm_cols(gi, s));
+    }
+    for (const auto& lv : g.l2l_levels) {
+        GemmCols gl{Sn, lv.ncol, 8, 1, plan->dop.o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
+                    g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
+        HIPCK(launch_gemm_cols(gl, s));
+    }
+    return FMM3DPP_OK;
+}
+
+// L2P (fmm.cpp:231-257, kernel.cpp:1458-1511): q_down = 0.5^depth * VSinv_l2p * (UT_l2p * p_check),
+// then potential and gradient at the leaves' bodies (written, not accumulated).
+int stage_l2p_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t* leaf_depth, int nleaf) {
+    hipStream_t s = plan->stream;
+    const int Sn = plan->ops.S;
+    const int32_t* iota = plan->gs.iota.as<int32_t>();
+    GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_utl.as<double>(), 0, plan->d_pck.as<double>(), leaf_cell, iota, 0,
+                plan->d_pcleaf2.as<double>(), 0, nullptr};
+    g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
+    HIPCK(launch_gemm_cols(g1, s));
+    GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
+                plan->d_pcleaf.as<double>(), 0, leaf_depth};
+    g2.K = plan->dop.pinv_rank;
+    HIPCK(launch_gemm_cols(g2, s));
+    HIPCK(launch_l2p_eval(Sn, plan->dop.o_usurf.as<double>(), leaf_cell, nleaf, plan->cells_dev(), plan->bodies_dev(),
+                          plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
+                          plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
+    return FMM3DPP_OK;
+}
+
+}  // namespace fmm3dpp
+
+namespace {
 
 // Device tree/lists plus the per-evaluate work buffers sized for them.
 // with_lists = false leaves the near-field lists to GpuStructure::build_lists (fmm3dpp_solve_device).
@@ -173,16 +256,6 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
     return FMM3DPP_OK;
 }
 
-int ensure_host_structure(fmm3dpp_plan* plan) {
-    if (plan->host_struct_valid) return FMM3DPP_OK;
-    if (plan->device < 0 || !plan->particles) FAIL(FMM3DPP_ERR_STATE, "no particles");
-    HIPCK(hipSetDevice(plan->device));
-    int rc = plan->gs.export_structure(plan->cfg, plan->st, plan->stream, plan->err);
-    if (rc) return rc;
-    plan->host_struct_valid = true;
-    return FMM3DPP_OK;
-}
-
 int launch_near(fmm3dpp_plan* plan);
 
 // Far field of an evaluation (fmm.cpp:50-74 up to L2P): gather, P2M, M2M, M2L, L2L, L2P, on the
@@ -190,116 +263,22 @@ int launch_near(fmm3dpp_plan* plan);
 // built, so the P2P is queued on the side stream right after the M2L.
 int eval_far(fmm3dpp_plan* plan, const double* d_qin, bool near_now) {
     hipStream_t s = plan->stream;
-    const Operators& o = plan->ops;
     const GpuStructure& g = plan->gs;
-    const int n = (int)g.n, Sn = o.S, nleaf = g.nleaf;
-    const CellsDev cells = plan->cells_dev();
-    const BodiesDev bod = plan->bodies_dev();
-    double* qeq = plan->d_qeq.as<double>();
-    double* pck = plan->d_pck.as<double>();
-    const size_t nc = (size_t)g.ncells;
-    const size_t SS = (size_t)Sn * Sn;
-    const int32_t* iota = g.iota.as<int32_t>();
-
+    const int Sn = plan->ops.S;
     HIPCK(hipEventRecord(plan->ev[0], s));
-    HIPCK(launch_gather(d_qin, g.perm.as<int32_t>(), plan->d_q.as<double>(), n, s));
-    HIPCK(hipMemsetAsync(pck, 0, nc * Sn * sizeof(double), s));
-    // P2M (fmm.cpp:83-109): check potentials, then q_equiv = 0.5^depth * VSinv * (UT * p_check)
-    HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
-                           plan->d_pcleaf.as<double>(), s));
-    {
-        GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
-                    plan->d_pcleaf2.as<double>(), 0, nullptr};
-        g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-        HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota,
-                    g.leaf_cell.as<int32_t>(), 0, qeq, 0, g.leaf_depth.as<int32_t>()};
-        g2.K = plan->dop.pinv_rank;
-        HIPCK(launch_gemm_cols(g2, s));
-    }
+    HIPCK(launch_gather(d_qin, g.perm.as<int32_t>(), plan->d_q.as<double>(), (int)g.n, s));
+    HIPCK(hipMemsetAsync(plan->d_pck.p, 0, (size_t)g.ncells * Sn * sizeof(double), s));
+    int rc = stage_p2m_leaves(plan, g.leaf_cell.as<int32_t>(), g.leaf_depth.as<int32_t>(), g.nleaf);
+    if (rc) return rc;
     HIPCK(hipEventRecord(plan->ev[1], s));
-    // M2M (fmm.cpp:111-150): per level, one GEMM per child octant into partials, then the child sum
-    for (const auto& lv : g.m2m_levels) {
-        GemmCols gm{Sn, lv.ncol, 8, 0, plan->dop.o_m2m_all.as<double>(), (int64_t)SS, qeq,
-                    g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol, plan->d_m2m_part.as<double>(), 0, nullptr};
-        HIPCK(launch_gemm_cols(gm, s));
-        HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
-                                  qeq, s));
-    }
-    // infinite-lattice gauge: the root and every image cell carry the zero monopole of a neutral box
-    const bool neutral = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
-    if (neutral) HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, s));
-    for (size_t i = 0; i < g.img_m2m.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
-                    g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
-        HIPCK(launch_gemm_cols(gi, s));
-        if (neutral) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, s));
-    }
+    if ((rc = stage_m2m_all(plan))) return rc;
     HIPCK(hipEventRecord(plan->ev[2], s));
-    // M2L (fmm.cpp:152-193, kernel.cpp:1140-1397)
-    // Targets = sources = the tree's non-leaf cells [0, nl), then the image cells [nl, ntar).
-    const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;
-    const int32_t* slots = g.child_slots.as<int32_t>();
-    const int32_t* inv = plan->dop.o_invlat.as<int32_t>();
-    const double* tw = plan->dop.o_tw.as<double>();
-    if (!plan->mixed()) {
-        HIPCK(launch_m2l_up(o.P, false, Sn, ntar, slots, inv, tw, qeq, plan->d_Qk.p, s));
-        HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(false, o.F, ntar, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
-                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
-        HIPCK(hipEventRecord(plan->ev[4], s));
-        HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, 0,
-                                plan->dop.o_gk27.as<double>(), plan->d_Qk.as<double>(), plan->d_Pk.as<double>(), s));
-        HIPCK(launch_m2l_down(o.P, false, Sn, ntar, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
-    } else {
-        // float32 spectra and Hadamard for the tree; the image lattice stays float64 end to end
-        HIPCK(launch_m2l_up(o.P, true, Sn, nl, slots, inv, tw, qeq, plan->d_Qk.p, s));
-        HIPCK(hipEventRecord(plan->ev[3], s));
-        HIPCK(launch_hadamard8(true, o.F, nl, ntar, g.slot_src_T.as<int32_t>(), g.had(), plan->dop.o_gkA.p,
-                               plan->d_Qk.p, plan->d_Pk.p, s, plan->had_cap));
-        HIPCK(hipEventRecord(plan->ev[4], s));
-        HIPCK(launch_m2l_down(o.P, true, Sn, nl, slots, g.tar_scale.as<double>(), inv, tw, plan->d_Pk.p, pck, s));
-        if (nimg > 0) {
-            HIPCK(launch_m2l_up(o.P, false, Sn, nimg, slots + (size_t)nl * 8, inv, tw, qeq, plan->d_Qk_img.p, s));
-            HIPCK(launch_hadamard27(o.F, nimg, g.img_tar.as<int32_t>(), g.img_src_T.as<int32_t>(), ntar, nl,
-                                    plan->dop.o_gk27.as<double>(), plan->d_Qk_img.as<double>(),
-                                    plan->d_Pk_img.as<double>(), s));
-            HIPCK(launch_m2l_down(o.P, false, Sn, nimg, slots + (size_t)nl * 8, g.tar_scale.as<double>() + nl, inv,
-                                  tw, plan->d_Pk_img.p, pck, s));
-        }
-    }
+    if ((rc = stage_m2l_all(plan))) return rc;
     HIPCK(hipEventRecord(plan->ev[5], s));
-    if (near_now) {
-        int rc = launch_near(plan);
-        if (rc) return rc;
-    }
-    // L2L (fmm.cpp:195-229): image levels from the outermost image, then one launch per tree level
-    const size_t nimgm2m = g.img_m2m.size();
-    for (size_t i = 0; i < g.img_l2l.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 1, plan->dop.o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + nimgm2m + i,
-                    g.img_b.as<int32_t>() + nimgm2m + i, 0, pck, 0, nullptr};
-        HIPCK(launch_gemm_cols(gi, s));
-    }
-    for (const auto& lv : g.l2l_levels) {
-        GemmCols gl{Sn, lv.ncol, 8, 1, plan->dop.o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
-                    g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
-        HIPCK(launch_gemm_cols(gl, s));
-    }
+    if (near_now && (rc = launch_near(plan))) return rc;
+    if ((rc = stage_l2l_all(plan))) return rc;
     HIPCK(hipEventRecord(plan->ev[6], s));
-    // L2P (fmm.cpp:231-257): q_down = 0.5^depth * VSinv_l2p * (UT_l2p * p_check), then evaluation
-    {
-        GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
-                    plan->d_pcleaf2.as<double>(), 0, nullptr};
-        g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-        HIPCK(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
-                    plan->d_pcleaf.as<double>(), 0, g.leaf_depth.as<int32_t>()};
-        g2.K = plan->dop.pinv_rank;
-        HIPCK(launch_gemm_cols(g2, s));
-    }
-    HIPCK(launch_l2p_eval(Sn, plan->dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
-                          plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
-                          plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
+    if ((rc = stage_l2p_leaves(plan, g.leaf_cell.as<int32_t>(), g.leaf_depth.as<int32_t>(), g.nleaf))) return rc;
     HIPCK(hipEventRecord(plan->ev[7], s));
     return FMM3DPP_OK;
 }
@@ -436,6 +415,7 @@ int fmm3dpp_set_particles(fmm3dpp_plan* plan, int64_t n, const double* x, const 
     if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
     plan->particles = false;
     plan->host_struct_valid = false;
+    plan->host_lists_valid = false;
     if (plan->device < 0) {
         int rc = build_structure(plan->cfg, n, x, y, z, plan->st, plan->err);
         if (rc) return rc;
@@ -463,6 +443,7 @@ int fmm3dpp_set_particles_device(fmm3dpp_plan* plan, int64_t n, const double* d_
     if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
     plan->particles = false;
     plan->host_struct_valid = false;
+    plan->host_lists_valid = false;
     HIPCK(hipSetDevice(plan->device));
     int rc = build_device(plan, n, d_x, d_y, d_z);
     if (rc) return rc;
@@ -491,6 +472,7 @@ int fmm3dpp_solve_device(fmm3dpp_plan* plan, int64_t n, const double* d_x, const
     if (n < 1 || n > (int64_t)INT32_MAX / 2) FAIL(FMM3DPP_ERR_ARG, "number of bodies must be in [1, 2^30)");
     plan->particles = false;
     plan->host_struct_valid = false;
+    plan->host_lists_valid = false;
     HIPCK(hipSetDevice(plan->device));
     int rc = ensure_ops(plan);
     if (rc) return rc;

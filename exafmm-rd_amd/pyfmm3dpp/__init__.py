@@ -31,6 +31,9 @@ EXPORTS = (
     "fmm3dpp_batch_create", "fmm3dpp_batch_destroy", "fmm3dpp_batch_last_error",
     "fmm3dpp_batch_set_particles_device", "fmm3dpp_batch_evaluate_device", "fmm3dpp_batch_set_graph",
     "fmm3dpp_batch_autotune", "fmm3dpp_batch_timing_ms",
+    "fmm3dpp_stage_charges", "fmm3dpp_stage_p2m", "fmm3dpp_stage_m2m", "fmm3dpp_stage_m2l", "fmm3dpp_stage_l2l",
+    "fmm3dpp_stage_l2p", "fmm3dpp_stage_p2p", "fmm3dpp_stage_finalize", "fmm3dpp_get_cell_vectors",
+    "fmm3dpp_set_cell_vectors", "fmm3dpp_get_body_fields", "fmm3dpp_set_body_fields",
 )
 
 
@@ -115,6 +118,16 @@ def lib():
         L.fmm3dpp_batch_autotune.argtypes = [vp, _ip, ctypes.c_int32, vp, vp, vp, _ip]
         L.fmm3dpp_batch_timing_ms.argtypes = [vp, ctypes.c_char_p]
         L.fmm3dpp_batch_timing_ms.restype = ctypes.c_double
+        L.fmm3dpp_stage_charges.argtypes = [vp, _dp]
+        for f in ("fmm3dpp_stage_p2m", "fmm3dpp_stage_m2m", "fmm3dpp_stage_l2l", "fmm3dpp_stage_l2p"):
+            getattr(L, f).argtypes = [vp, ctypes.c_int64, _ip]
+        L.fmm3dpp_stage_m2l.argtypes = [vp]
+        L.fmm3dpp_stage_p2p.argtypes = [vp, ctypes.c_int64, _ip, ctypes.c_int]
+        L.fmm3dpp_stage_finalize.argtypes = [vp, _dp, _dp]
+        L.fmm3dpp_get_cell_vectors.argtypes = [vp, ctypes.c_int, ctypes.c_int64, _ip, _dp]
+        L.fmm3dpp_set_cell_vectors.argtypes = [vp, ctypes.c_int, ctypes.c_int64, _ip, _dp]
+        L.fmm3dpp_get_body_fields.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, _dp, _dp]
+        L.fmm3dpp_set_body_fields.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, _dp, _dp]
         _lib = L
     return _lib
 
@@ -290,6 +303,64 @@ class LaplaceFMM:
     def set_operator(self, name, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         self._check(lib().fmm3dpp_set_operator(self.h, name.encode(), _d(arr), arr.size))
+
+    # --- per-stage entry points (LaplaceKernel's operator API) ---------------
+    def _cells(self, cells):
+        if cells is None:
+            return 0, None
+        a = np.ascontiguousarray(cells, dtype=np.int32)
+        self._keep = a
+        return len(a), _i(a)
+
+    def stage_charges(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._check(lib().fmm3dpp_stage_charges(self.h, _d(q)))
+
+    def stage(self, name, cells=None, accumulate=1):
+        """One far- or near-field stage: "p2m", "m2m", "m2l", "l2l", "l2p", "p2p" over the listed cells
+        (None: every cell the stage covers, as evaluate)."""
+        L = lib()
+        if name == "m2l":
+            self._check(L.fmm3dpp_stage_m2l(self.h))
+            return
+        n, p = self._cells(cells)
+        if name == "p2p":
+            self._check(L.fmm3dpp_stage_p2p(self.h, n, p, int(accumulate)))
+        else:
+            self._check(getattr(L, f"fmm3dpp_stage_{name}")(self.h, n, p))
+
+    def stage_finalize(self):
+        phi = np.empty(self.n)
+        grad = np.empty((self.n, 3))
+        self._check(lib().fmm3dpp_stage_finalize(self.h, _d(phi), _d(grad)))
+        return phi, grad
+
+    def cell_vectors(self, which, cells=None):
+        """which 0: q_equiv, 1: p_check of the listed cells (None: all), shape (ncells, S)."""
+        c = self.counts()
+        n, p = self._cells(cells)
+        out = np.empty((c["n_cells"] if cells is None else n, c["S"]))
+        self._check(lib().fmm3dpp_get_cell_vectors(self.h, which, n, p, _d(out)))
+        return out
+
+    def set_cell_vectors(self, which, values, cells=None):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        n, p = self._cells(cells)
+        self._check(lib().fmm3dpp_set_cell_vectors(self.h, which, n, p, _d(v)))
+
+    def body_fields(self, first=0, count=None):
+        """Potential and gradient of bodies [first, first + count) in tree order, before the dipole
+        term and scaling."""
+        count = self.n - first if count is None else count
+        phi = np.empty(count)
+        grad = np.empty((count, 3))
+        self._check(lib().fmm3dpp_get_body_fields(self.h, first, count, _d(phi), _d(grad)))
+        return phi, grad
+
+    def set_body_fields(self, phi, grad, first=0):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self._check(lib().fmm3dpp_set_body_fields(self.h, first, len(phi), _d(phi), _d(grad)))
 
     def stage_vectors(self):
         c = self.counts()

@@ -147,6 +147,52 @@ int fmm3dpp_set_operator(fmm3dpp_plan* plan, const char* name, const double* in,
 /* Per-cell equivalent densities and check potentials [n_cells][S] after the last evaluate. */
 int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_check);
 
+/* ---- per-stage entry points: the operator API of the reference's LaplaceKernel ----
+ * Stages act on the plan's state in HBM after set_particles: per-cell vectors q_equiv and p_check
+ * ([n_cells][S], cells numbered as fmm3dpp_get_cells) and per-body fields (potential and gradient,
+ * before the dipole term and scaling).  `cells` lists the cells a stage acts on; NULL = every cell
+ * it covers, as evaluate.  fmm3dpp_stage_charges, then p2m, m2m, m2l, l2l, l2p, p2p (accumulate = 1)
+ * and finalize with NULL lists give evaluate's results bit for bit.  Listed cells of one tree level
+ * run at once; levels run in evaluate's order (M2M from the deepest, L2L from the outermost image
+ * cell down).  Host arrays; every call returns after its work.  Replaced interfaces (include/kernel.h,
+ * src/kernel.cpp of the reference's 3dpp/):
+ *   fmm3dpp_stage_charges   <- the charges of Bodies3 the operators read
+ *   fmm3dpp_stage_p2m       <- p2m_precompute (kernel.h:43, kernel.cpp:332-371): q_equiv of the listed
+ *                              leaves = 0.5^depth VSinv UT p_check(bodies) (written; with cells = NULL
+ *                              every p_check is also cleared, as evaluate does first)
+ *   fmm3dpp_stage_m2m       <- m2m_precompute / m2m_img_precompute (kernel.h:47-51, kernel.cpp:395-442):
+ *                              q_equiv of the listed parents = sum over children of M2M[octant] q_equiv
+ *                              (image cells: the 27 image operators summed), written; in the
+ *                              infinite-lattice gauge the root and image cells get a zero monopole
+ *   fmm3dpp_stage_m2l       <- m2l_fft_precompute_t (kernel.h:97, kernel.cpp:1140-1397): p_check +=
+ *                              M2L over the plan's M2L parent map (= Traverser::get_M2L_parent_map)
+ *   fmm3dpp_stage_l2l       <- l2l_precompute / l2l_img_precompute (kernel.h:101-103,
+ *                              kernel.cpp:1419-1435): children's p_check += L2L of the listed parents
+ *   fmm3dpp_stage_l2p       <- l2p_precompute (kernel.h:107, kernel.cpp:1458-1511): the body fields of
+ *                              the listed leaves = the field of 0.5^depth VSinv_l2p UT_l2p p_check
+ *   fmm3dpp_stage_p2p       <- p2p_1toN_256 (kernel.h:39, kernel.cpp:258-311) over the plan's P2P list
+ *                              of each listed target leaf (= Traverser::get_map(P2P)); accumulate = 0
+ *                              writes the near field, 1 adds it to the body fields
+ *   fmm3dpp_stage_finalize  <- dipole_correction + scale_bodies + sort_bodies_by_idx (fmm.cpp:75-80):
+ *                              phi[n], grad[n][3] in input order from the body fields
+ * The precompute of kernel.h:113-115 is fmm3dpp_precompute. */
+int fmm3dpp_stage_charges(fmm3dpp_plan* plan, const double* q);
+int fmm3dpp_stage_p2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells);
+int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells);
+int fmm3dpp_stage_m2l(fmm3dpp_plan* plan);
+int fmm3dpp_stage_l2l(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells);
+int fmm3dpp_stage_l2p(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells);
+int fmm3dpp_stage_p2p(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells, int accumulate);
+int fmm3dpp_stage_finalize(fmm3dpp_plan* plan, double* phi, double* grad);
+/* Per-cell vectors of the listed cells (NULL: all n_cells), which = 0 q_equiv, 1 p_check; out / in
+ * hold ncells x S values (the Cell3::q_equiv / p_check members of the reference, include/tree.h:9-30). */
+int fmm3dpp_get_cell_vectors(fmm3dpp_plan* plan, int which, int64_t ncells, const int32_t* cells, double* out);
+int fmm3dpp_set_cell_vectors(fmm3dpp_plan* plan, int which, int64_t ncells, const int32_t* cells, const double* in);
+/* Body fields of bodies [first, first + count) in tree order (fmm3dpp_get_body_order; a cell's body
+ * range from fmm3dpp_get_cells): phi[count], grad[count][3]. */
+int fmm3dpp_get_body_fields(fmm3dpp_plan* plan, int64_t first, int64_t count, double* phi, double* grad);
+int fmm3dpp_set_body_fields(fmm3dpp_plan* plan, int64_t first, int64_t count, const double* phi, const double* grad);
+
 /* ---- operator cache (the reference recomputes its operators on every solve,
  *      src/kernel.cpp:1638-1850) ----
  * save writes the plan's operators (computing them first if needed) to `path`; load replaces the

@@ -298,6 +298,8 @@ struct Plan {
     std::vector<int> up_idx, down_idx;
     // per-cell vectors
     std::vector<double> qeq, pck;
+    // charges and body fields in tree order (kept between the per-stage calls)
+    std::vector<double> q, p, gx, gy, gz;
     std::map<std::string, double> timers;
 };
 
@@ -1022,33 +1024,23 @@ int orc_set_particles(void* h, int n, const double* x, const double* y, const do
 }
 
 // LaplaceFMM::solve stage sequence (fmm.cpp:50-80).  Outputs in input order.
-int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* gyo, double* gzo) {
-    Plan* pl = (Plan*)h;
-    if (pl->nthreads > 0) omp_set_num_threads(pl->nthreads);
-    if (!pl->have_ops) { pl->err = "operators not computed"; return -1; }
-    int n = pl->n, S = pl->S;
-    size_t C = pl->cells.size();
-    std::vector<double> q(n), p(n, 0.0), gx(n, 0.0), gy(n, 0.0), gz(n, 0.0);
-    for (int i = 0; i < n; i++) q[i] = qin[pl->bidx[i]];
-    pl->qeq.assign(C * S, 0.0);
-    pl->pck.assign(C * S, 0.0);
-    int mind = 0, maxd = 0;
-    for (auto& c : pl->cells) { mind = std::min(mind, c.depth); maxd = std::max(maxd, c.depth); }
+static void depth_range(const Plan& pl, int& mind, int& maxd) {
+    mind = 0; maxd = 0;
+    for (auto& c : pl.cells) { mind = std::min(mind, c.depth); maxd = std::max(maxd, c.depth); }
+}
+
+static std::vector<int> leaf_cells(const Plan& pl) {
     std::vector<int> leaves;
-    for (auto& c : pl->cells) if (c.cnum == 0) leaves.push_back(c.idx);
-    double t0 = now();
-    stage_p2m(*pl, leaves, q.data());
-    double t1 = now();
-    stage_m2m(*pl, mind, maxd, pl->dipole == 2);
-    double t2 = now();
-    stage_m2l(*pl);
-    double t3 = now();
-    stage_l2l(*pl, mind, maxd);
-    double t4 = now();
-    stage_l2p(*pl, leaves, p.data(), gx.data(), gy.data(), gz.data());
-    double t5 = now();
-    stage_p2p(*pl, q.data(), p.data(), gx.data(), gy.data(), gz.data());
-    double t6 = now();
+    for (auto& c : pl.cells) if (c.cnum == 0) leaves.push_back(c.idx);
+    return leaves;
+}
+
+// dipole_correction (kernel.cpp:1580-1596) or the infinite-lattice gauge, scale_bodies (body.cpp:88-96),
+// and input order (fmm.cpp:80), from the body fields in tree order.
+static void finalize(Plan* pl, double* pout, double* gxo, double* gyo, double* gzo) {
+    const int n = pl->n;
+    std::vector<double> p = pl->p, gx = pl->gx, gy = pl->gy, gz = pl->gz;
+    const std::vector<double>& q = pl->q;
     if (pl->dipole == 2) {   // infinite-lattice gauge (no counterpart in the reference)
         const double coef = 4 * M_PI / (3 * pl->cycle * pl->cycle * pl->cycle);
         double M[4] = {0, 0, 0, 0};
@@ -1078,9 +1070,102 @@ int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* 
         int o = pl->bidx[i];
         pout[o] = p[i]; gxo[o] = gx[i]; gyo[o] = gy[i]; gzo[o] = gz[i];
     }
+}
+
+// Charges in tree order; per-cell vectors and body fields cleared.
+static void set_charges(Plan* pl, const double* qin) {
+    const int n = pl->n;
+    const size_t C = pl->cells.size();
+    pl->q.resize(n);
+    for (int i = 0; i < n; i++) pl->q[i] = qin[pl->bidx[i]];
+    pl->p.assign(n, 0.0); pl->gx.assign(n, 0.0); pl->gy.assign(n, 0.0); pl->gz.assign(n, 0.0);
+    pl->qeq.assign(C * pl->S, 0.0);
+    pl->pck.assign(C * pl->S, 0.0);
+}
+
+// LaplaceFMM::solve stage sequence (fmm.cpp:50-80).  Outputs in input order.
+int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* gyo, double* gzo) {
+    Plan* pl = (Plan*)h;
+    if (pl->nthreads > 0) omp_set_num_threads(pl->nthreads);
+    if (!pl->have_ops) { pl->err = "operators not computed"; return -1; }
+    set_charges(pl, qin);
+    int mind = 0, maxd = 0;
+    depth_range(*pl, mind, maxd);
+    const std::vector<int> leaves = leaf_cells(*pl);
+    double t0 = now();
+    stage_p2m(*pl, leaves, pl->q.data());
+    double t1 = now();
+    stage_m2m(*pl, mind, maxd, pl->dipole == 2);
+    double t2 = now();
+    stage_m2l(*pl);
+    double t3 = now();
+    stage_l2l(*pl, mind, maxd);
+    double t4 = now();
+    stage_l2p(*pl, leaves, pl->p.data(), pl->gx.data(), pl->gy.data(), pl->gz.data());
+    double t5 = now();
+    stage_p2p(*pl, pl->q.data(), pl->p.data(), pl->gx.data(), pl->gy.data(), pl->gz.data());
+    double t6 = now();
+    finalize(pl, pout, gxo, gyo, gzo);
     pl->timers["P2M"] = t1 - t0; pl->timers["M2M"] = t2 - t1; pl->timers["M2L"] = t3 - t2;
     pl->timers["L2L"] = t4 - t3; pl->timers["L2P"] = t5 - t4; pl->timers["P2P"] = t6 - t5;
     pl->timers["FMM_kernels"] = now() - t0;
+    return 0;
+}
+
+// One stage of orc_evaluate at a time (the checker of the library's per-stage entry points):
+// "charges" (qin, input order; clears the per-cell vectors and body fields), "p2m", "m2m", "m2l",
+// "l2l", "l2p" (body fields = far field), "p2p" (body fields += near field), "p2p_only" (body
+// fields = near field).  Each stage reads the state the previous calls left.
+int orc_stage(void* h, const char* name, const double* qin) {
+    Plan* pl = (Plan*)h;
+    if (pl->nthreads > 0) omp_set_num_threads(pl->nthreads);
+    if (!pl->have_ops) { pl->err = "operators not computed"; return -1; }
+    const std::string s(name ? name : "");
+    if (s == "charges") {
+        if (!qin) return -1;
+        set_charges(pl, qin);
+        return 0;
+    }
+    if ((int)pl->q.size() != pl->n) { pl->err = "charges not set"; return -1; }
+    int mind = 0, maxd = 0;
+    depth_range(*pl, mind, maxd);
+    if (s == "p2m") stage_p2m(*pl, leaf_cells(*pl), pl->q.data());
+    else if (s == "m2m") stage_m2m(*pl, mind, maxd, pl->dipole == 2);
+    else if (s == "m2l") stage_m2l(*pl);
+    else if (s == "l2l") stage_l2l(*pl, mind, maxd);
+    else if (s == "l2p" || s == "p2p_only") {
+        const int n = pl->n;
+        pl->p.assign(n, 0.0); pl->gx.assign(n, 0.0); pl->gy.assign(n, 0.0); pl->gz.assign(n, 0.0);
+        if (s == "l2p") stage_l2p(*pl, leaf_cells(*pl), pl->p.data(), pl->gx.data(), pl->gy.data(), pl->gz.data());
+        else stage_p2p(*pl, pl->q.data(), pl->p.data(), pl->gx.data(), pl->gy.data(), pl->gz.data());
+    } else if (s == "p2p") stage_p2p(*pl, pl->q.data(), pl->p.data(), pl->gx.data(), pl->gy.data(), pl->gz.data());
+    else { pl->err = "unknown stage " + s; return -1; }
+    return 0;
+}
+
+int orc_stage_finalize(void* h, double* pout, double* gxo, double* gyo, double* gzo) {
+    Plan* pl = (Plan*)h;
+    if ((int)pl->q.size() != pl->n) return -1;
+    finalize(pl, pout, gxo, gyo, gzo);
+    return 0;
+}
+
+// Body fields in tree order.
+int orc_get_body_fields(void* h, double* p, double* gx, double* gy, double* gz) {
+    Plan* pl = (Plan*)h;
+    if ((int)pl->p.size() != pl->n) return -1;
+    std::memcpy(p, pl->p.data(), pl->n * sizeof(double));
+    std::memcpy(gx, pl->gx.data(), pl->n * sizeof(double));
+    std::memcpy(gy, pl->gy.data(), pl->n * sizeof(double));
+    std::memcpy(gz, pl->gz.data(), pl->n * sizeof(double));
+    return 0;
+}
+
+int orc_set_stage_vectors(void* h, const double* qeq, const double* pck) {
+    Plan* pl = (Plan*)h;
+    const size_t m = pl->cells.size() * pl->S;
+    if (qeq) pl->qeq.assign(qeq, qeq + m);
+    if (pck) pl->pck.assign(pck, pck + m);
     return 0;
 }
 

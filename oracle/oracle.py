@@ -55,6 +55,10 @@ def lib():
         L.orc_get_operator.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.c_long]
         L.orc_set_operator.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.c_long]
         L.orc_get_stage_vectors.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_set_stage_vectors.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+        L.orc_stage_finalize.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_get_body_fields.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_drand48_bodies.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_direct.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int,
@@ -168,6 +172,34 @@ class OracleFMM:
         arr = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         if lib().orc_set_operator(self.h, name.encode(), _d(arr), arr.size) != 0:
             raise KeyError(name)
+
+    def stage(self, name, q=None):
+        """One stage of evaluate on the state the previous calls left: "charges" (q, input order),
+        "p2m", "m2m", "m2l", "l2l", "l2p" (body fields = far field), "p2p" (+= near field),
+        "p2p_only" (body fields = near field)."""
+        if not self._ops:
+            self.precompute()
+        qa = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+        if lib().orc_stage(self.h, name.encode(), None if qa is None else _d(qa)) != 0:
+            raise RuntimeError(lib().orc_last_error(self.h).decode())
+
+    def stage_finalize(self):
+        p, gx, gy, gz = (np.empty(self.n) for _ in range(4))
+        if lib().orc_stage_finalize(self.h, _d(p), _d(gx), _d(gy), _d(gz)) != 0:
+            raise RuntimeError("charges not set")
+        return p, np.stack([gx, gy, gz], axis=1)
+
+    def body_fields(self):
+        """Potential and gradient of the bodies in tree order before the dipole term and scaling."""
+        p, gx, gy, gz = (np.empty(self.n) for _ in range(4))
+        if lib().orc_get_body_fields(self.h, _d(p), _d(gx), _d(gy), _d(gz)) != 0:
+            raise RuntimeError("charges not set")
+        return p, np.stack([gx, gy, gz], axis=1)
+
+    def set_stage_vectors(self, qeq=None, pck=None):
+        a = None if qeq is None else np.ascontiguousarray(qeq, dtype=np.float64)
+        b = None if pck is None else np.ascontiguousarray(pck, dtype=np.float64)
+        lib().orc_set_stage_vectors(self.h, None if a is None else _d(a), None if b is None else _d(b))
 
     def stage_vectors(self):
         S = lib().orc_surface_num(self.h)

@@ -144,3 +144,20 @@ def test_batch_config_checks():
         assert L.fmm3dpp_batch_destroy(h) == pyfmm3dpp.OK
     assert L.fmm3dpp_batch_destroy(None) == pyfmm3dpp.ERR_ARG
     assert L.fmm3dpp_batch_timing_ms(None, b"evaluate") == -1
+
+
+def test_stage_entry_points_need_a_device_plan():
+    """The per-stage entry points act on device state: a structure-only plan refuses them."""
+    import pyfmm3dpp
+    L = pyfmm3dpp.lib()
+    plan = pyfmm3dpp.LaplaceFMM(4, 16, 1, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    rng = np.random.default_rng(1)
+    p = rng.random((200, 3)) - 0.5
+    plan.set_particles(p[:, 0], p[:, 1], p[:, 2])
+    q = np.zeros(200)
+    assert L.fmm3dpp_stage_charges(plan.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == pyfmm3dpp.ERR_DEVICE
+    for f in ("fmm3dpp_stage_p2m", "fmm3dpp_stage_m2m", "fmm3dpp_stage_l2l", "fmm3dpp_stage_l2p"):
+        assert getattr(L, f)(plan.h, 0, None) == pyfmm3dpp.ERR_DEVICE
+    assert L.fmm3dpp_stage_m2l(plan.h) == pyfmm3dpp.ERR_DEVICE
+    assert L.fmm3dpp_stage_p2p(plan.h, 0, None, 1) == pyfmm3dpp.ERR_DEVICE
+    assert L.fmm3dpp_stage_m2l(None) == pyfmm3dpp.ERR_ARG

@@ -88,3 +88,23 @@ def test_fmm_approximates_finite_lattice(oracle_mod, P, images, tol_f):
     l2p = np.linalg.norm(p[tar] - pd) / np.linalg.norm(pd)
     l2f = np.linalg.norm(g[tar] - gd) / np.linalg.norm(gd)
     assert l2f < tol_f and l2p < tol_f, (l2p, l2f)
+
+
+@pytest.mark.parametrize("dipole", [1, 2])
+def test_oracle_stages_equal_evaluate(oracle_mod, dipole):
+    """The oracle's per-stage form (the checker of the library's per-stage entry points) runs the
+    same code as its evaluate: the stage sequence reproduces evaluate bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(3))
+    n = 3000
+    x = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    f = oracle_mod.OracleFMM(6, 32, 2, 1.0, dipole=dipole, threads=4)
+    f.precompute()
+    f.set_particles(x[:, 0], x[:, 1], x[:, 2])
+    p0, g0 = f.evaluate(q)
+    f.stage("charges", q)
+    for s in ("p2m", "m2m", "m2l", "l2l", "l2p", "p2p"):
+        f.stage(s)
+    p1, g1 = f.stage_finalize()
+    assert np.array_equal(p0, p1) and np.array_equal(g0, g1)
