@@ -278,7 +278,7 @@ struct Range {   // the reference's OffsetAndNumber (include/type.h:550-565)
     Range(int o, int n) : offset(o), number(n) {}
 };
 
-struct Cell3 {   // include/tree.h:9-30 without the per-cell q_equiv / p_check (kept in HBM here)
+struct Cell3 {   // include/tree.h:9-30
     int idx = 0;
     int octant = 0;   // 0..7 below a parent, 13 for the single child of an image cell
     int depth = 0;
@@ -286,6 +286,10 @@ struct Cell3 {   // include/tree.h:9-30 without the per-cell q_equiv / p_check (
     vec3r x;
     Range crange;
     Range brange;
+    // Equivalent density and check potential (S values each once an operator has written them; empty
+    // reads as zeros).  LaplaceFMM keeps them in HBM; LaplaceKernel's per-cell operators read and
+    // write them here, as the reference's Matrix members.
+    std::vector<real> q_equiv, p_check;
 };
 using Cells3 = std::vector<Cell3>;
 
@@ -463,12 +467,169 @@ private:
     PeriodicM2LMap m2l_;
 };
 
-// LaplaceKernel::direct (src/kernel.cpp:11-77) on the GPU: bs_tar.p/f += the direct lattice sum of
-// bs_src over (3^images)^3 copies; sources and targets are the same bodies, as the reference's
-// test driver calls it.
+// LaplaceKernel (include/kernel.h:15-115) on the GPU.
+//  * direct (src/kernel.cpp:11-77): bs_tar.p/f += the direct lattice sum of bs_src over (3^images)^3
+//    copies; sources and targets are the same bodies, as the reference's test driver calls it.
+//  * p2p (src/kernel.cpp:79-100): one cell pair.
+//  * the operator API of the evaluation path (precompute, p2m_precompute, m2m_precompute,
+//    m2m_img_precompute, m2l_fft_precompute_t, l2l_precompute, l2l_img_precompute, l2p_precompute,
+//    p2p_1toN_256): the same signatures and the same effect on the caller's Cell3::q_equiv / p_check
+//    and Body3::p / f, computed by the per-stage entry points of the C-ABI on a device plan.  The
+//    device needs the tree the cells come from, so a kernel used for these is constructed with the
+//    run's Argument (ncrit, box centre, cycle, device); the first call that passes the bodies builds
+//    the same tree on the GPU from them (bodies in tree order, as Tree::build leaves them) and checks
+//    every cell it is given against it.  Differences to the reference, all in its scratch use:
+//    p2m_precompute writes q_equiv (the reference adds the bodies' check potential to the q_equiv it
+//    finds, zero on its evaluation path, before the solve), and l2p_precompute leaves p_check
+//    unchanged (the reference overwrites it with the downward equivalent density).
 class LaplaceKernel {
 public:
     explicit LaplaceKernel(int device = 0) : device_(device) {}
+    explicit LaplaceKernel(const Argument& args)
+        : device_(args.device), ncrit_(args.ncrit), cycle_(args.cycle), center_(args.x), bound_args_(true) {}
+    ~LaplaceKernel() { if (plan_) fmm3dpp_plan_destroy(plan_); }
+    LaplaceKernel(const LaplaceKernel&) = delete;
+    LaplaceKernel& operator=(const LaplaceKernel&) = delete;
+
+    // precompute (kernel.h:113, src/kernel.cpp:1638-1704): the operators of (P, images) for a box of
+    // half size r0 (cycle 2 r0), M2L kernel spectra included.
+    void precompute(int P, real r0, int images) {
+        if (!bound_args_) throw std::runtime_error("LaplaceKernel: construct with the run's Argument for the operator API");
+        if (images >= 1 && std::fabs(2 * r0 - cycle_) > 1e-12 * cycle_)
+            throw std::runtime_error("LaplaceKernel::precompute: r0 must be half the Argument's cycle");
+        if (plan_) fmm3dpp_plan_destroy(plan_);
+        plan_ = nullptr;
+        fmm3dpp_config cfg{};
+        cfg.P = P;
+        cfg.ncrit = ncrit_;
+        cfg.images = images;
+        cfg.cycle = 2 * r0;
+        for (int d = 0; d < 3; d++) cfg.center[d] = center_[d];
+        cfg.precision = FMM3DPP_PREC_FP64;
+        cfg.dipole_mode = FMM3DPP_DIPOLE_OFF;
+        int rc = fmm3dpp_plan_create(&cfg, device_, &plan_);
+        if (rc != FMM3DPP_OK) throw std::runtime_error("fmm3dpp_plan_create failed: " + std::to_string(rc));
+        check(fmm3dpp_precompute(plan_));
+        P_ = P;
+        bodies_ = nullptr;
+        nbodies_ = 0;
+        S_ = 6 * (P - 1) * (P - 1) + 2;
+    }
+    // precompute_m2l (kernel.h:115, src/kernel.cpp:1706-1850): the M2L spectra of every relative
+    // position are part of precompute here; this checks the arguments only.
+    void precompute_m2l(int P, real r0, Cells3 cs, PeriodicInteractionMap m2l_map, int images) {
+        (void)cs; (void)m2l_map; (void)r0; (void)images;
+        need(P);
+    }
+
+    // p2m_precompute (kernel.h:43, src/kernel.cpp:332-371): cell_src.q_equiv = 0.5^depth VSinv UT
+    // (check potential of the leaf's bodies on its 2.95 r surface).
+    void p2m_precompute(int P, Bodies3& bs_src, Cell3& cell_src) {
+        need(P);
+        bind(bs_src, true);
+        const int32_t c = leaf(cell_src);
+        check(fmm3dpp_stage_p2m(plan_, 1, &c));
+        cell_src.q_equiv.resize(S_);
+        check(fmm3dpp_get_cell_vectors(plan_, 0, 1, &c, cell_src.q_equiv.data()));
+    }
+    // m2m_precompute (kernel.h:47, src/kernel.cpp:395-404): parent.q_equiv += sum of M2M[octant]
+    // child.q_equiv over the parent's children in cs.
+    void m2m_precompute(int P, Cell3& cell_parent, Cells3& cs) {
+        need(P);
+        upward(cell_parent, cs, false);
+    }
+    // m2m_img_precompute (kernel.h:51, src/kernel.cpp:434-442): image cell parent.q_equiv += the 27
+    // image M2M operators applied to its single child's q_equiv.  `cycle` (the image cell's box, as
+    // the reference's driver passes it) does not enter: the operators depend on ratios of radii only.
+    void m2m_img_precompute(int P, Cell3& cell_parent, Cells3& cs, real cycle) {
+        (void)cycle;
+        need(P);
+        upward(cell_parent, cs, true);
+    }
+    // m2l_fft_precompute_t (kernel.h:97, src/kernel.cpp:1140-1397): every cell's p_check += M2L over
+    // the M2L parent map of the tree (Traverser::get_M2L_parent_map of the same tree, checked by size).
+    void m2l_fft_precompute_t(int P, Cells3& cs, PeriodicM2LMap& m2l_parent_map) {
+        need(P);
+        need_bound();
+        if ((int64_t)cs.size() != counts_.n_cells) throw std::runtime_error("m2l: cs must hold every cell, image cells included");
+        int64_t rows = 0;
+        for (const auto& kv : m2l_parent_map) rows += (int64_t)kv.second.size();
+        if (rows != counts_.n_m2l) throw std::runtime_error("m2l: the M2L parent map is not the tree's");
+        std::vector<real> qe((size_t)S_ * cs.size(), 0.0), pc((size_t)S_ * cs.size(), 0.0);
+        for (size_t c = 0; c < cs.size(); c++) {
+            if (cs[c].q_equiv.size() == (size_t)S_) std::copy(cs[c].q_equiv.begin(), cs[c].q_equiv.end(), &qe[c * S_]);
+            if (cs[c].p_check.size() == (size_t)S_) std::copy(cs[c].p_check.begin(), cs[c].p_check.end(), &pc[c * S_]);
+        }
+        check(fmm3dpp_set_cell_vectors(plan_, 0, 0, nullptr, qe.data()));
+        check(fmm3dpp_set_cell_vectors(plan_, 1, 0, nullptr, pc.data()));
+        check(fmm3dpp_stage_m2l(plan_));
+        check(fmm3dpp_get_cell_vectors(plan_, 1, 0, nullptr, pc.data()));
+        for (size_t c = 0; c < cs.size(); c++) cs[c].p_check.assign(&pc[c * S_], &pc[(c + 1) * S_]);
+    }
+    // l2l_precompute (kernel.h:101, src/kernel.cpp:1419-1428): each child's p_check += L2L[octant]
+    // parent.p_check.
+    void l2l_precompute(int P, Cell3& cell_parent, Cells3& cs) {
+        need(P);
+        downward(cell_parent, cs);
+    }
+    // l2l_img_precompute (kernel.h:103, src/kernel.cpp:1430-1435): the single child's p_check +=
+    // L2L_IMG parent.p_check.
+    void l2l_img_precompute(int P, Cell3& cell_parent, Cells3& cs) {
+        need(P);
+        downward(cell_parent, cs);
+    }
+    // l2p_precompute (kernel.h:107, src/kernel.cpp:1458-1511): bodies of cell_tar get p += and f +=
+    // the field of q_down = 0.5^depth VSinv_l2p UT_l2p p_check at their positions.
+    void l2p_precompute(int P, Bodies3& bs_tar, Cell3& cell_tar) {
+        need(P);
+        bind(bs_tar, false);
+        const int32_t c = leaf(cell_tar);
+        std::vector<real> pc = cell_tar.p_check;
+        pc.resize(S_, 0.0);
+        check(fmm3dpp_set_cell_vectors(plan_, 1, 1, &c, pc.data()));
+        check(fmm3dpp_stage_l2p(plan_, 1, &c));
+        add_fields(bs_tar, cell_tar);
+    }
+    // p2p_1toN_256 (kernel.h:39, src/kernel.cpp:258-311): bodies of cell_tar in bs_tar get p += sum q/r
+    // and f += sum -q dx/r^3 over the bodies of every (cell, offset) of p2ps in bs_src, shifted by the
+    // offset (r = 0 skipped): one direct sum on the GPU over exactly the listed sources.
+    void p2p_1toN_256(Bodies3& bs_src, Bodies3& bs_tar, Cells3& cs, std::vector<std::pair<int, vec3r>>& p2ps,
+                      Cell3& cell_tar) {
+        const int nt = cell_tar.brange.number;
+        if (nt <= 0) return;
+        if (cell_tar.brange.offset < 0 || cell_tar.brange.offset + nt > (int)bs_tar.size())
+            throw std::runtime_error("p2p_1toN_256: target body range out of range");
+        std::vector<real> x, y, z, q;
+        for (const auto& e : p2ps) {
+            const Cell3& c = cs.at(e.first);
+            if (c.brange.offset < 0 || c.brange.offset + c.brange.number > (int)bs_src.size())
+                throw std::runtime_error("p2p_1toN_256: source body range out of range");
+            for (int i = 0; i < c.brange.number; i++) {
+                const Body3& b = bs_src[c.brange.offset + i];
+                x.push_back(b.x[0] + e.second[0]);
+                y.push_back(b.x[1] + e.second[1]);
+                z.push_back(b.x[2] + e.second[2]);
+                q.push_back(b.q);
+            }
+        }
+        const int ns = (int)x.size();
+        std::vector<int32_t> tar(nt);
+        for (int j = 0; j < nt; j++) {   // the targets, with zero charge
+            const Body3& b = bs_tar[cell_tar.brange.offset + j];
+            x.push_back(b.x[0]); y.push_back(b.x[1]); z.push_back(b.x[2]); q.push_back(0.0);
+            tar[j] = ns + j;
+        }
+        std::vector<real> phi(nt), grad(3 * (size_t)nt);
+        int rc = fmm3dpp_direct(device_, (int64_t)x.size(), x.data(), y.data(), z.data(), q.data(), nt, tar.data(), 1.0,
+                                0, phi.data(), grad.data());
+        if (rc != FMM3DPP_OK) throw std::runtime_error("fmm3dpp_direct failed: " + std::to_string(rc));
+        for (int j = 0; j < nt; j++) {
+            Body3& b = bs_tar[cell_tar.brange.offset + j];
+            b.p += phi[j];
+            for (int d = 0; d < 3; d++) b.f[d] += grad[3 * (size_t)j + d];
+        }
+    }
+
     void direct(const Bodies3& bs_src, Bodies3& bs_tar, int images, real cycle) {
         if (&bs_src != &bs_tar && bs_src.size() != bs_tar.size())
             throw std::runtime_error("direct: sources and targets must be the same bodies");
@@ -520,7 +681,123 @@ public:
     }
 
 private:
+    void check(int rc) {
+        if (rc != FMM3DPP_OK)
+            throw std::runtime_error(std::string("fmm3dpp: ") + (plan_ ? fmm3dpp_last_error(plan_) : "no plan"));
+    }
+    void need_plan() {
+        if (!plan_) throw std::runtime_error("LaplaceKernel: precompute must come first");
+    }
+    void need(int P) {
+        need_plan();
+        if (P != P_) throw std::runtime_error("LaplaceKernel: P differs from the precompute's");
+    }
+    void need_bound() {
+        if (!bodies_) throw std::runtime_error("LaplaceKernel: no bodies yet (p2m_precompute binds them)");
+    }
+    // The first call with a body set builds the device tree and lists from it; a body set of another
+    // address or size rebuilds.  charges: upload the charges (they may change between P2M and P2P).
+    void bind(const Bodies3& bs, bool charges) {
+        if (&bs != bodies_ || bs.size() != nbodies_) {
+            std::vector<real> x, y, z, q;
+            detail::soa(bs, x, y, z, q);
+            check(fmm3dpp_set_particles(plan_, (int64_t)bs.size(), x.data(), y.data(), z.data()));
+            check(fmm3dpp_get_counts(plan_, &counts_));
+            std::vector<int32_t> perm(bs.size());
+            check(fmm3dpp_get_body_order(plan_, perm.data()));
+            for (size_t i = 0; i < perm.size(); i++)
+                if (perm[i] != (int32_t)i) throw std::runtime_error("LaplaceKernel: bodies must be in tree order (Tree::build)");
+            ints_.assign(7 * (size_t)counts_.n_cells, 0);
+            std::vector<double> dbl(4 * (size_t)counts_.n_cells);
+            check(fmm3dpp_get_cells(plan_, ints_.data(), dbl.data()));
+            bodies_ = &bs;
+            nbodies_ = bs.size();
+            charges = true;
+        }
+        if (charges) {
+            std::vector<real> x, y, z, q;
+            detail::soa(bs, x, y, z, q);
+            check(fmm3dpp_stage_charges(plan_, q.data()));
+        }
+    }
+    // The cell must be the device tree's cell of the same index (same body range, children).
+    int32_t cell_index(const Cell3& c) {
+        need_bound();
+        if (c.idx < 0 || c.idx >= counts_.n_cells) throw std::runtime_error("LaplaceKernel: cell index out of range");
+        const int32_t* I = &ints_[7 * (size_t)c.idx];
+        if (I[2] != c.depth || I[3] != c.crange.offset || I[4] != c.crange.number || I[5] != c.brange.offset ||
+            I[6] != c.brange.number)
+            throw std::runtime_error("LaplaceKernel: cell does not belong to the tree of these bodies");
+        return c.idx;
+    }
+    int32_t leaf(const Cell3& c) {
+        const int32_t i = cell_index(c);
+        if (c.crange.number != 0) throw std::runtime_error("LaplaceKernel: not a leaf cell");
+        return i;
+    }
+    static std::vector<real> padded(const std::vector<real>& v, int S) {
+        std::vector<real> out = v;
+        out.resize(S, 0.0);
+        return out;
+    }
+    // M2M of one parent from its children's q_equiv in cs, added to parent.q_equiv.
+    void upward(Cell3& parent, Cells3& cs, bool image) {
+        const int32_t p = cell_index(parent);
+        if (image != (parent.depth < 0)) throw std::runtime_error("m2m: image operator on a tree cell or the reverse");
+        for (int k = 0; k < parent.crange.number; k++) {
+            const Cell3& ch = cs.at(parent.crange.offset + k);
+            const int32_t c = cell_index(ch);
+            std::vector<real> v = padded(ch.q_equiv, S_);
+            check(fmm3dpp_set_cell_vectors(plan_, 0, 1, &c, v.data()));
+        }
+        check(fmm3dpp_stage_m2m(plan_, 1, &p));
+        std::vector<real> r(S_);
+        check(fmm3dpp_get_cell_vectors(plan_, 0, 1, &p, r.data()));
+        parent.q_equiv.resize(S_, 0.0);
+        for (int i = 0; i < S_; i++) parent.q_equiv[i] += r[i];
+    }
+    // L2L of one parent: children's p_check (from cs) += L2L parent.p_check.
+    void downward(Cell3& parent, Cells3& cs) {
+        const int32_t p = cell_index(parent);
+        std::vector<real> v = padded(parent.p_check, S_);
+        check(fmm3dpp_set_cell_vectors(plan_, 1, 1, &p, v.data()));
+        std::vector<int32_t> kids;
+        for (int k = 0; k < parent.crange.number; k++) {
+            Cell3& ch = cs.at(parent.crange.offset + k);
+            const int32_t c = cell_index(ch);
+            std::vector<real> w = padded(ch.p_check, S_);
+            check(fmm3dpp_set_cell_vectors(plan_, 1, 1, &c, w.data()));
+            kids.push_back(c);
+        }
+        check(fmm3dpp_stage_l2l(plan_, 1, &p));
+        for (int k = 0; k < parent.crange.number; k++) {
+            Cell3& ch = cs[parent.crange.offset + k];
+            ch.p_check.resize(S_);
+            check(fmm3dpp_get_cell_vectors(plan_, 1, 1, &kids[k], ch.p_check.data()));
+        }
+    }
+    void add_fields(Bodies3& bs, const Cell3& c) {
+        const int n = c.brange.number;
+        std::vector<real> phi(n), grad(3 * (size_t)n);
+        check(fmm3dpp_get_body_fields(plan_, c.brange.offset, n, phi.data(), grad.data()));
+        for (int i = 0; i < n; i++) {
+            Body3& b = bs[c.brange.offset + i];
+            b.p += phi[i];
+            for (int d = 0; d < 3; d++) b.f[d] += grad[3 * (size_t)i + d];
+        }
+    }
+
     int device_;
+    int ncrit_ = 0;
+    real cycle_ = 0;
+    vec3r center_;
+    bool bound_args_ = false;
+    fmm3dpp_plan* plan_ = nullptr;
+    int P_ = 0, S_ = 0;
+    const Bodies3* bodies_ = nullptr;
+    size_t nbodies_ = 0;
+    fmm3dpp_counts counts_{};
+    std::vector<int32_t> ints_;
 };
 
 // EwaldSolver (include/ewald.h:20-21, src/ewald.cpp:6-203) on the GPU with the reference's

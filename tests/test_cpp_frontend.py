@@ -48,3 +48,21 @@ def test_tree_and_traverser_match_binding(prog, oracle_mod, n, ncrit, images):
     assert got == want
     c = plan.counts()
     assert f"cells {c['n_cells']} p2p {c['n_p2p']}" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,P,ncrit,images", [(3000, 6, 32, 2), (2500, 4, 16, 3)])
+def test_kernel_operator_api_equals_solve(n, P, ncrit, images):
+    """The stage loops of LaplaceFMM::solve (3dpp/src/fmm.cpp:83-300) over the drop-in LaplaceKernel's
+    per-cell operators (p2m_precompute, m2m(_img)_precompute, m2l_fft_precompute_t,
+    l2l(_img)_precompute, l2p_precompute, p2p_1toN_256) equal the drop-in LaplaceFMM::solve: the
+    operators run the same kernels per cell; the P2P is a direct sum over the listed sources (a
+    different summation order) and the dipole term is summed on the host, so 1e-12."""
+    exe = os.path.join(ROOT, "tests", "cpp", "kernel_api")
+    assert os.path.exists(exe), "build with make -C exafmm-rd_amd/csrc"
+    print(f"kernel API n={n} P={P} ncrit={ncrit} images={images}", flush=True)
+    r = subprocess.run([exe, str(n), str(P), str(ncrit), str(images)], capture_output=True, text=True, timeout=300)
+    print(r.stdout.strip(), flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    assert float(f[1]) <= 1e-12 and float(f[3]) <= 1e-12
