@@ -59,19 +59,64 @@ def hadamard_flops(plan):
     return 8.0 * 64.0 * c["n_freq"] * pairs, pairs
 
 
-def cpu_baseline(x, y, z, q, P, ncrit, images, threads, budget_s):
-    """The CPU restatement (oracle/, C++/OpenMP) timed on this host's cores on the same inputs."""
+def cpu_info():
+    """Model, sockets and cores of this host from lscpu (empty fields if lscpu is missing)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        k, v = k.strip(), v.strip()
+        if k == "Model name":
+            info["model"] = v
+        elif k == "Socket(s)":
+            info["sockets"] = int(v)
+        elif k == "Core(s) per socket":
+            info["cores_per_socket"] = int(v)
+        elif k == "Thread(s) per core":
+            info["threads_per_core"] = int(v)
+    return info
+
+
+def cpu_baseline(x, y, z, q, P, ncrit, images, threads, repeats=3):
+    """The CPU restatement (oracle/, C++17 + OpenMP, built -O3 -march=x86-64-v3) timed on this host's
+    cores on the same inputs: one full step (tree + lists + every stage, precompute excluded), median
+    of `repeats`, on `threads` threads and again on 8 (the reference's logged runs use 8).  Its
+    fidelity is checked in the same run: the energy of the reference's N=24000 drand48 fixture
+    (P=8, 3 image levels, ncrit 128, box 2 pi) against the logged 2.758374398027e+01."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    f = oracle.OracleFMM(P, ncrit, images, 1.0, dipole=1, threads=threads)
-    f.precompute()
-    t0 = time.perf_counter()
-    f.set_particles(x, y, z)
-    f.evaluate(q)
-    dt = time.perf_counter() - t0
-    return {"value": len(x) / dt, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": f"one full step (tree+lists+all stages, precompute excluded) of the same N={len(x)} workload, "
-                      f"{dt:.1f} s", "seconds": round(dt, 3)}
+
+    def timed(nthreads):
+        f = oracle.OracleFMM(P, ncrit, images, 1.0, dipole=1, threads=nthreads)
+        f.precompute()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            f.set_particles(x, y, z)
+            f.evaluate(q)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
+    dt, ts = timed(threads)
+    dt8, ts8 = timed(8)
+    fx, fy, fz, fq = oracle.drand48_bodies(24000, np.pi, seed=5, zero_netcharge=1)
+    fix = oracle.OracleFMM(8, 128, 3, 2 * np.pi, dipole=1, threads=threads)
+    fix.precompute()
+    fix.set_particles(fx, fy, fz)
+    fp, _ = fix.evaluate(fq)
+    energy = float(np.sum(fq * fp))
+    logged = 2.758374398027e+01   # 3dpp/exp/slurm-153086.out:2899
+    return {"value": round(len(x) / dt, 1), "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"one full step (tree + lists + all stages, precompute excluded) of the same N={len(x)} "
+                      f"workload, median of {repeats}",
+            "seconds": [round(t, 3) for t in ts],
+            "threads8": {"value": round(len(x) / dt8, 1), "seconds": [round(t, 3) for t in ts8]},
+            "host": cpu_info(),
+            "fixture_energy": {"value": energy, "logged": logged, "rel_diff": float("%.2e" % (abs(energy - logged) / logged))}}
 
 
 def plummer_workload(n, seed):
@@ -479,7 +524,7 @@ def main():
         log(f"batch leg: {out['batch']['speedup']}x over one plan per system")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        out["cpu_baseline"] = cpu_baseline(x, y, z, q, args.P, args.ncrit, args.images, threads, 30)
+        out["cpu_baseline"] = cpu_baseline(x, y, z, q, args.P, args.ncrit, args.images, threads)
         log(f"CPU baseline {out['cpu_baseline']['value']:.3g} particles/s on {threads} threads")
     print(json.dumps(out), flush=True)
 

@@ -397,14 +397,24 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
         BHIP(launch_gemm_cols(gm, s));
         BHIP(launch_sum_partials(Sn, lv.ncol, 8, g.m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off, qeq, s));
     }
-    // infinite-lattice gauge: every system's root and image cells carry the zero monopole of a neutral box
+    // infinite-lattice gauge: every system's root and image cells get their charge as monopole (the
+    // system's total charge times 27^m for c(-m)); the per-system moments are those of the finalize
     const bool neutral = b->cfg[0].dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
-    if (neutral) BHIP(launch_neutralize(qeq, Sn, g.roots.as<int32_t>(), 0, (int)g.sys.size(), s));
+    const int nsys_g = (int)g.sys.size();
+    if (neutral) {
+        BHIP(launch_dipole_seg(bod, g.seg_off.as<int64_t>(), nsys_g, kDipoleBlocks, g.partial.as<double>(),
+                               g.moments.as<double>(), s));
+        BHIP(launch_neutralize(qeq, Sn, g.roots.as<int32_t>(), 0, nsys_g, g.moments.as<double>() + 4, 5, 1.0, s));
+    }
+    double img_scale = 1.0;
     for (const auto& lv : g.img_m2m_lv) {
         GemmCols gi{Sn, (int)lv[1], 1, 0, g.dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + lv[0],
                     g.img_a.as<int32_t>() + lv[0], 0, qeq, 0, nullptr};
         BHIP(launch_gemm_cols(gi, s));
-        if (neutral) BHIP(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + lv[0], 0, (int)lv[1], s));
+        img_scale *= 27.0;
+        if (neutral)
+            BHIP(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + lv[0], 0, (int)lv[1], g.moments.as<double>() + 4, 5,
+                                   img_scale, s));
     }
     // M2L: targets = sources = all systems' non-leaf cells [0, nl), then all image cells [nl, ntar)
     const int ntar = g.ntar, nl = g.nl, nimg = g.nimg;

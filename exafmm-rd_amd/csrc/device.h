@@ -109,8 +109,11 @@ hipError_t launch_p2p_f32(const int32_t* leaf_cell, int nleaf, CellsDev cells, c
                           const int64_t* list_off, const int32_t* list_ent, double cycle, double* p, double* gx,
                           double* gy, double* gz, bool accumulate, hipStream_t st);
 // Infinite-lattice gauge: shift the equivalent density of cells[0..count) (first..first+count without a
-// list) so that its sum is zero (the monopole of a neutral box).
-hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first, int count, hipStream_t st);
+// list) uniformly so that its sum (monopole) is scale * qsum[i * qstride] for the i-th listed cell.
+hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first, int count, const double* qsum,
+                             int qstride, double scale, hipStream_t st);
+// moments[0..5) = the sum of partial's nblocks rows of 5 (launch_dipole_partial's), in row order.
+hipError_t launch_moments(const double* partial, int nblocks, double* moments, hipStream_t st);
 // partial[5 * block + (0..4)] = sum q x, sum q y, sum q z, sum q |x|^2, sum q
 hipError_t launch_dipole_partial(BodiesDev b, int n, double* partial /* 5 * nblocks */, int nblocks, hipStream_t st);
 // dL/dq and dL/dx from w = dL/dphi, grad phi, and (psi, grad psi) = the evaluation with charges w

@@ -166,21 +166,24 @@ __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, c
     Y[(int64_t)out[c] * S + r] = a;
 }
 
-// Infinite-lattice gauge (no counterpart in the reference): the equivalent density of a neutral
-// box's root carries a monopole of ~1e-10 of its size (the rounding the pseudo-inverse leaves); each
-// image level copies it 27 times and the 1x27 lattice M2L turns it into a potential offset that
-// grows 9x per image level.  Each listed cell's density is shifted uniformly so that its sum, the
-// cell's monopole, is zero.  One wave per cell, cell = cells[block] (or first + block without a
-// list); lane-strided partial sums, then an xor butterfly, after which every lane holds the same sum.
+// Infinite-lattice gauge (no counterpart in the reference): the root's equivalent density carries a
+// monopole that differs from the bodies' total charge by ~1e-10 of the density's size (the rounding
+// the pseudo-inverse leaves); each image level copies it 27 times and the 1x27 lattice M2L turns it
+// into a potential offset that grows 9x per image level.  Each listed cell's density is shifted
+// uniformly so that its sum, the cell's monopole, equals its charge: scale * qsum[block * qstride]
+// (the total charge of the block's system times 27^m for the image cell c(-m)).  One wave per cell,
+// cell = cells[block] (or first + block without a list); lane-strided partial sums, then an xor
+// butterfly, after which every lane holds the same sum.
 __global__ __launch_bounds__(64) void k_neutralize(double* __restrict__ qeq, int S, const int32_t* __restrict__ cells,
-                                                   int first) {
+                                                   int first, const double* __restrict__ qsum, int qstride,
+                                                   double scale) {
     const int c = cells ? cells[blockIdx.x] : first + (int)blockIdx.x;
     if (c < 0) return;   // uniform over the wave
     double* v = qeq + (int64_t)c * S;
     double a = 0;
     for (int j = threadIdx.x; j < S; j += 64) a += v[j];
     for (int m = 32; m > 0; m >>= 1) a += __shfl_xor(a, m);
-    const double shift = -a / S;
+    const double shift = (scale * qsum[(int64_t)blockIdx.x * qstride] - a) / S;
     for (int j = threadIdx.x; j < S; j += 64) v[j] += shift;
 }
 
@@ -385,10 +388,17 @@ hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, cons
     return hipGetLastError();
 }
 
-hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first, int count, hipStream_t st) {
+hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first, int count, const double* qsum,
+                             int qstride, double scale, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    if (!qeq || S <= 0 || (!cells && first < 0)) return hipErrorInvalidValue;
-    k_neutralize<<<count, 64, 0, st>>>(qeq, S, cells, first);
+    if (!qeq || !qsum || S <= 0 || qstride < 0 || (!cells && first < 0)) return hipErrorInvalidValue;
+    k_neutralize<<<count, 64, 0, st>>>(qeq, S, cells, first, qsum, qstride, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(const double* partial, int nblocks, double* moments, hipStream_t st) {
+    if (!partial || !moments || nblocks <= 0) return hipErrorInvalidValue;
+    k_moments_seg<<<1, 64, 0, st>>>(partial, nblocks, moments);
     return hipGetLastError();
 }
 

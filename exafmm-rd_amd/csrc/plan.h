@@ -96,6 +96,9 @@ int ensure_host_structure(fmm3dpp_plan* plan);
 // plan's device tree.  p2m / l2p act on the leaves leaf_cell[0..nleaf) (depths leaf_depth) and write
 // q_equiv / the per-body fields of those leaves; m2m, m2l and l2l cover every cell as evaluate does.
 int stage_p2m_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t* leaf_depth, int nleaf);
+// The bodies' total charge on the device (the moments of launch_dipole_partial, summed into d_partial's
+// row 256); *qtot points at it.
+int total_charge(fmm3dpp_plan* plan, const double** qtot);
 int stage_m2m_all(fmm3dpp_plan* plan);
 int stage_m2l_all(fmm3dpp_plan* plan);
 int stage_l2l_all(fmm3dpp_plan* plan);

@@ -96,9 +96,20 @@ int stage_p2m_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t
     return FMM3DPP_OK;
 }
 
+int total_charge(fmm3dpp_plan* plan, const double** qtot) {
+    hipStream_t s = plan->stream;
+    const int nblk = 256;
+    double* part = plan->d_partial.as<double>();
+    HIPCK(launch_dipole_partial(plan->bodies_dev(), (int)plan->gs.n, part, nblk, s));
+    HIPCK(launch_moments(part, nblk, part + (size_t)nblk * 5, s));
+    *qtot = part + (size_t)nblk * 5 + 4;
+    return FMM3DPP_OK;
+}
+
 // M2M (fmm.cpp:111-150, kernel.cpp:395-404, 434-442): per tree level from the deepest, one GEMM per
 // child octant into partials and the child sum; then the image levels.  In the infinite-lattice
-// gauge the root and every image cell get the zero monopole of a neutral box.
+// gauge the root's and every image cell's monopole are set to their charge (the bodies' total times
+// 27^m for c(-m)), removing the rounding-level monopole the pseudo-inverse leaves.
 int stage_m2m_all(fmm3dpp_plan* plan) {
     hipStream_t s = plan->stream;
     const GpuStructure& g = plan->gs;
@@ -113,13 +124,17 @@ int stage_m2m_all(fmm3dpp_plan* plan) {
         HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
                                   qeq, s));
     }
-    const bool neutral = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
-    if (neutral) HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, s));
+    const double* qtot = nullptr;   // the bodies' total charge (infinite-lattice gauge)
+    if (plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL) {
+        int rc = total_charge(plan, &qtot);
+        if (rc) return rc;
+        HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, qtot, 0, 1.0, s));
+    }
     for (size_t i = 0; i < g.img_m2m.size(); i++) {
         GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
                     g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
         HIPCK(launch_gemm_cols(gi, s));
-        if (neutral) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, s));
+        if (qtot) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, qtot, 0, std::pow(27.0, i + 1), s));
     }
     return FMM3DPP_OK;
 }

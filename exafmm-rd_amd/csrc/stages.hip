@@ -148,7 +148,8 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
     const int Sn = plan->ops.S;
     const size_t SS = (size_t)Sn * Sn;
     const size_t chunk = (size_t)std::max(1, plan->gs.max_level_cols);
-    const bool neutral = plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL;
+    const double* qtot = nullptr;   // the bodies' total charge (infinite-lattice gauge)
+    if (plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL && (rc = total_charge(plan, &qtot))) return rc;
     double* qeq = plan->d_qeq.as<double>();
     hipStream_t s = plan->stream;
     for (auto it = lv.rbegin(); it != lv.rend(); ++it) {   // deepest level first
@@ -167,7 +168,8 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
                 HIPCK(launch_gemm_cols(gm, s));
                 HIPCK(launch_sum_partials(Sn, (int)nc, 8, plan->d_m2m_part.as<double>(), plan->d_sel.as<int32_t>(), qeq,
                                           s));
-                if (neutral && it->first == 0) HIPCK(launch_neutralize(qeq, Sn, plan->d_sel.as<int32_t>(), 0, (int)nc, s));
+                if (qtot && it->first == 0)
+                    HIPCK(launch_neutralize(qeq, Sn, plan->d_sel.as<int32_t>(), 0, (int)nc, qtot, 0, 1.0, s));
                 HIPCK(hipStreamSynchronize(s));   // the tables are reused by the next chunk
             }
         } else {
@@ -179,7 +181,9 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
                 gi.in_idx = plan->d_sel_in.as<int32_t>();
                 gi.out_idx = plan->d_sel_out.as<int32_t>();
                 HIPCK(launch_gemm_cols(gi, s));
-                if (neutral) HIPCK(launch_neutralize(qeq, Sn, plan->d_sel_out.as<int32_t>(), 0, 1, s));
+                if (qtot)
+                    HIPCK(launch_neutralize(qeq, Sn, plan->d_sel_out.as<int32_t>(), 0, 1, qtot, 0,
+                                            std::pow(27.0, -plan->st.cells[p].depth), s));
                 HIPCK(hipStreamSynchronize(s));
             }
         }

@@ -47,7 +47,11 @@ extern "C" {
                                    *   grad_i -= (4 pi / 3V) M
                                    * which turns the lattice sum over the (3^images)^3 cube into the
                                    * Ewald potential of a neutral system (the cube's dipole sum is
-                                   * conditionally convergent; de Leeuw, Perram and Smith 1980).    */
+                                   * conditionally convergent; de Leeuw, Perram and Smith 1980).
+                                   * The equivalent densities of the root and of each image cell
+                                   * c(-m) get their charge, 27^m sum_j q_j, as monopole: the
+                                   * pseudo-inverse leaves ~1e-10 of their size, which the image
+                                   * levels would multiply by 9 per level into a potential offset. */
 
 #define FMM3DPP_DEVICE_NONE (-1)  /* structure-only plan: tree, lists and operators on the host; */
                                   /* evaluate returns FMM3DPP_ERR_DEVICE                          */
@@ -163,7 +167,7 @@ int fmm3dpp_get_stage_vectors(fmm3dpp_plan* plan, double* q_equiv, double* p_che
  *   fmm3dpp_stage_m2m       <- m2m_precompute / m2m_img_precompute (kernel.h:47-51, kernel.cpp:395-442):
  *                              q_equiv of the listed parents = sum over children of M2M[octant] q_equiv
  *                              (image cells: the 27 image operators summed), written; in the
- *                              infinite-lattice gauge the root and image cells get a zero monopole
+ *                              infinite-lattice gauge the root's and image cells' monopole is set to their charge
  *   fmm3dpp_stage_m2l       <- m2l_fft_precompute_t (kernel.h:97, kernel.cpp:1140-1397): p_check +=
  *                              M2L over the plan's M2L parent map (= Traverser::get_M2L_parent_map)
  *   fmm3dpp_stage_l2l       <- l2l_precompute / l2l_img_precompute (kernel.h:101-103,

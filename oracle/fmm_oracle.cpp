@@ -687,12 +687,13 @@ static void stage_p2m(Plan& pl, const std::vector<int>& leaves, const double* q)
 }
 
 // fmm.cpp:111-150 with kernel.cpp:395-404 and 434-442.
-// neutral = true in the infinite-lattice gauge (dipole mode 2, no counterpart in the reference): the
-// root's and every image cell's equivalent density are shifted uniformly so that their sum (the
-// cell's monopole) is zero, the charge of a neutral box.  The pseudo-inverse leaves a monopole of
-// ~1e-10 of the density's size at the root; each image level copies it 27 times and the 1x27
-// lattice M2L turns it into a potential offset growing 9x per image level.
-static void stage_m2m(Plan& pl, int mind, int maxd, bool neutral = false) {
+// qtot != null selects the infinite-lattice gauge (dipole mode 2, no counterpart in the reference):
+// the root's and every image cell's equivalent density are shifted uniformly so that their sum (the
+// cell's monopole) equals the cell's charge, 27^m *qtot for the image cell c(-m).  The
+// pseudo-inverse leaves a monopole that differs from the charge by ~1e-10 of the density's size;
+// each image level copies it 27 times and the 1x27 lattice M2L turns it into a potential offset
+// growing 9x per image level.
+static void stage_m2m(Plan& pl, int mind, int maxd, const double* qtot = nullptr) {
     int S = pl.S;
     for (int d = maxd; d > mind; d--) {
         std::vector<int> cs;
@@ -715,10 +716,10 @@ static void stage_m2m(Plan& pl, int mind, int maxd, bool neutral = false) {
                     for (int s = 0; s < S; s++) qp[s] += tmp[s];
                 }
             }
-            if (neutral && d <= 0) {
+            if (qtot && d <= 0) {
                 double sum = 0;
                 for (int s = 0; s < S; s++) sum += qp[s];
-                const double shift = -sum / S;
+                const double shift = (*qtot * std::pow(27.0, -d) - sum) / S;
                 for (int s = 0; s < S; s++) qp[s] += shift;
             }
         }
@@ -1095,7 +1096,9 @@ int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* 
     double t0 = now();
     stage_p2m(*pl, leaves, pl->q.data());
     double t1 = now();
-    stage_m2m(*pl, mind, maxd, pl->dipole == 2);
+    double qsum = 0;
+    for (int i = 0; i < pl->n; i++) qsum += pl->q[i];
+    stage_m2m(*pl, mind, maxd, pl->dipole == 2 ? &qsum : nullptr);
     double t2 = now();
     stage_m2l(*pl);
     double t3 = now();
@@ -1130,7 +1133,11 @@ int orc_stage(void* h, const char* name, const double* qin) {
     int mind = 0, maxd = 0;
     depth_range(*pl, mind, maxd);
     if (s == "p2m") stage_p2m(*pl, leaf_cells(*pl), pl->q.data());
-    else if (s == "m2m") stage_m2m(*pl, mind, maxd, pl->dipole == 2);
+    else if (s == "m2m") {
+        double qsum = 0;
+        for (int i = 0; i < pl->n; i++) qsum += pl->q[i];
+        stage_m2m(*pl, mind, maxd, pl->dipole == 2 ? &qsum : nullptr);
+    }
     else if (s == "m2l") stage_m2l(*pl);
     else if (s == "l2l") stage_l2l(*pl, mind, maxd);
     else if (s == "l2p" || s == "p2p_only") {

@@ -88,9 +88,9 @@ def _share_operators(plan, orc, images):
 
 @pytest.mark.parametrize("images", [2, 6])
 def test_physical_gauge_matches_oracle(fmm, oracle_mod, images):
-    """FMM3DPP_DIPOLE_PHYSICAL against the oracle's infinite-lattice mode (zero monopole of the root
-    and image cells' equivalent densities, surface term of the cubic lattice sum) with shared
-    operators: float64, 1e-12 like every other parity case."""
+    """FMM3DPP_DIPOLE_PHYSICAL against the oracle's infinite-lattice mode (the root and image cells'
+    equivalent densities carry their charge as monopole, surface term of the cubic lattice sum) with
+    shared operators: float64, 1e-12 like every other parity case."""
     x, y, z, q = _uniform(8192, 21)
     plan = fmm.LaplaceFMM(8, 64, images, 1.0, dipole_mode=fmm.DIPOLE_PHYSICAL, device=0)
     plan.precompute()
@@ -108,8 +108,8 @@ def test_physical_gauge_matches_oracle(fmm, oracle_mod, images):
 def test_fmm_physical_gauge_vs_ewald(fmm, prec):
     """The lattice FMM in the infinite-lattice gauge converges to the Ewald sum as images grow (the
     remainder of a (3^i)^3 cube decays ~9x per level), float64 and mixed precision alike.  The raw
-    potential is compared (no constant removed): the zero monopole of the root and image cells keeps
-    the lattice free of the offset a rounding-level monopole would grow by 9x per image level.
+    potential is compared (no constant removed): setting the root and image cells' monopole to their
+    charge keeps the lattice free of the offset a rounding-level monopole would grow by 9x per level.
     Bounds: images=3 is the configuration's own lattice truncation (2.8e-4 measured on the CPU for
     the potential, 1.4e-5 for the gradient); from 5 levels on both are below 1e-5 and falling."""
     x, y, z, q = _uniform(16384, 33)

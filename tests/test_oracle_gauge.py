@@ -1,8 +1,9 @@
 """Pins the oracle's infinite-lattice gauge (dipole mode 2, no counterpart in the reference).
 
 Mode 2 is the raw lattice FMM with two changes: the equivalent densities of the root and of every
-image cell are shifted to a zero monopole (a neutral box), and the potential and gradient receive
-the surface term of the cubic lattice sum, phi -= (4 pi/3V) M.x - (2 pi/3V) sum q |x|^2,
+image cell are shifted so that their monopole equals their charge (the bodies' total, times 27^m for
+the image cell c(-m)), removing the rounding-level monopole the pseudo-inverse leaves, and the potential and gradient
+receive the surface term of the cubic lattice sum, phi -= (4 pi/3V) M.x - (2 pi/3V) sum q |x|^2,
 grad -= (4 pi/3V) M (de Leeuw, Perram and Smith 1980).  What pins it:
 
 * the tin-foil Ewald sum (the oracle's own, pinned against the reference's logged Ewald energy):
