@@ -127,6 +127,72 @@ static inline void gemv(const double* A, const double* x, double* y, int m, int 
     }
 }
 
+// ---- reduced-precision emulation (test infrastructure for the mixed-precision path) ----
+// EMUL_MONOPOLE: every tree cell's equivalent density is shifted uniformly so that its sum (the
+// monopole) equals the charge of the cell's bodies, after P2M (leaves) and M2M (parents).
+enum EmulBits { EMUL_P2M_CHECK = 1, EMUL_P2M_PINV = 2, EMUL_M2M = 4, EMUL_L2L = 8, EMUL_L2P_PINV = 16,
+                EMUL_L2P_EVAL = 32, EMUL_MONOPOLE = 64 };
+
+// Uniform shift of v[0..S) that makes its sum equal `charge`.
+static void set_monopole(double* v, int S, double charge) {
+    double sum = 0;
+    for (int s = 0; s < S; s++) sum += v[s];
+    const double shift = (charge - sum) / S;
+    for (int s = 0; s < S; s++) v[s] += shift;
+}
+// float32 operands and accumulation; split bfloat16 with 2 pieces and 3 products (hi*hi, hi*lo,
+// lo*hi), or 3 pieces and the 6 products of order <= 2, each product exact in float32 and
+// accumulated in float32 (the matrix-core split-product schemes).
+enum EmulKind { EMUL_F32 = 0, EMUL_BF16X3 = 1, EMUL_BF16X6 = 2 };
+
+// Round to the nearest bfloat16 (ties to even), returned as a float.
+static inline float bf16_round(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+
+static inline void bf16_split(double v, int pieces, float* out) {
+    double r = v;
+    for (int i = 0; i < pieces; i++) {
+        out[i] = bf16_round((float)r);
+        r -= out[i];
+    }
+}
+
+// y[i] = k * sum_{j<n} A[i*lda + j] x[j] in the arithmetic `kind` (EmulKind), rounded back to double.
+static void gemv_emul(const double* A, int lda, const double* x, double* y, int m, int n, double k, int kind) {
+    if (kind == EMUL_F32) {
+        std::vector<float> xf(n);
+        for (int j = 0; j < n; j++) xf[j] = (float)x[j];
+        for (int i = 0; i < m; i++) {
+            const double* a = A + (size_t)i * lda;
+            float s = 0;
+            for (int j = 0; j < n; j++) s += (float)a[j] * xf[j];
+            y[i] = k * (double)s;
+        }
+        return;
+    }
+    const int pieces = kind == EMUL_BF16X3 ? 2 : 3;
+    std::vector<float> xs((size_t)n * 3);
+    for (int j = 0; j < n; j++) bf16_split(x[j], pieces, &xs[(size_t)j * 3]);
+    for (int i = 0; i < m; i++) {
+        const double* a = A + (size_t)i * lda;
+        float s = 0;
+        for (int j = 0; j < n; j++) {
+            float as[3];
+            bf16_split(a[j], pieces, as);
+            const float* xj = &xs[(size_t)j * 3];
+            for (int u = 0; u < pieces; u++)
+                for (int v = 0; u + v < pieces; v++) s += as[u] * xj[v];
+        }
+        y[i] = k * (double)s;
+    }
+}
+
 // One-sided Jacobi SVD of a square n x n row-major A: A = U diag(s) V^T, s descending.
 // Stands in for LAPACK dgesvd_ (mathfunc.cpp:81-110).  Rotations are scheduled in
 // round-robin rounds of disjoint column pairs so each round runs in parallel.
@@ -281,6 +347,11 @@ struct Plan {
     double cycle, r0, x0[3];
     int S = 0, N = 0, N3 = 0, Nh = 0, F = 0;
     int m2l_bruteforce = 0;
+    // Reduced-precision emulation of the operator applies (no counterpart in the reference; models
+    // the error of a mixed-precision evaluation): emul_mask selects the stages (EMUL_* bits),
+    // emul_kind the arithmetic (EmulKind), pinv_rank > 0 applies the P2M and L2P pseudo-inverses
+    // at that rank (leading rows of UT / columns of VSinv), as the mixed path does.
+    int emul_mask = 0, emul_kind = 0, pinv_rank = 0;
     std::string err;
     // bodies in tree order
     int n = 0;
@@ -302,6 +373,19 @@ struct Plan {
     std::vector<double> q, p, gx, gy, gz;
     std::map<std::string, double> timers;
 };
+
+// y = k * A(m x n) x with A's leading dimension lda, in float64 or (stage bit set) the emulated arithmetic.
+static void gemv_stage(const Plan& pl, int bit, const double* A, int lda, const double* x, double* y, int m, int n,
+                       double k) {
+    if (pl.emul_mask & bit) { gemv_emul(A, lda, x, y, m, n, k, pl.emul_kind); return; }
+    if (lda == n) { gemv(A, x, y, m, n, k); return; }
+    for (int i = 0; i < m; i++) {
+        const double* a = A + (size_t)i * lda;
+        double s = 0;
+        for (int j = 0; j < n; j++) s += a[j] * x[j];
+        y[i] = k * s;
+    }
+}
 
 static double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -673,16 +757,34 @@ static void stage_p2m(Plan& pl, const std::vector<int>& leaves, const double* q)
         surface_points(pl.P, c.r * 2.95, c.x, chk);
         for (int s = 0; s < S; s++) {
             double acc = 0;
-            for (int b = c.boff; b < c.boff + c.bnum; b++) {
-                double dx = chk[3 * s] - pl.bx[b], dy = chk[3 * s + 1] - pl.by[b], dz = chk[3 * s + 2] - pl.bz[b];
-                double r2 = dx * dx + dy * dy + dz * dz;
-                acc += r2 > 0 ? q[b] * (1.0 / std::sqrt(r2)) : 0.0;
+            if (pl.emul_mask & EMUL_P2M_CHECK) {   // float32 cell-relative differences and sums
+                float af = 0;
+                for (int b = c.boff; b < c.boff + c.bnum; b++) {
+                    float dx = (float)(chk[3 * s] - c.x[0]) - (float)(pl.bx[b] - c.x[0]);
+                    float dy = (float)(chk[3 * s + 1] - c.x[1]) - (float)(pl.by[b] - c.x[1]);
+                    float dz = (float)(chk[3 * s + 2] - c.x[2]) - (float)(pl.bz[b] - c.x[2]);
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    af += r2 > 0 ? (float)q[b] * (1.0f / std::sqrt(r2)) : 0.0f;
+                }
+                acc = af;
+            } else {
+                for (int b = c.boff; b < c.boff + c.bnum; b++) {
+                    double dx = chk[3 * s] - pl.bx[b], dy = chk[3 * s + 1] - pl.by[b], dz = chk[3 * s + 2] - pl.bz[b];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    acc += r2 > 0 ? q[b] * (1.0 / std::sqrt(r2)) : 0.0;
+                }
             }
             pc[s] = acc;
         }
         double scale = std::pow(0.5, c.depth);
-        gemv(pl.UT.data(), pc.data(), buf.data(), S, S, 1.0);
-        gemv(pl.VSinv.data(), buf.data(), &pl.qeq[(size_t)c.idx * S], S, S, scale);
+        const int r = pl.pinv_rank > 0 ? std::min(pl.pinv_rank, S) : S;
+        gemv_stage(pl, EMUL_P2M_PINV, pl.UT.data(), S, pc.data(), buf.data(), r, S, 1.0);
+        gemv_stage(pl, EMUL_P2M_PINV, pl.VSinv.data(), S, buf.data(), &pl.qeq[(size_t)c.idx * S], S, r, scale);
+        if (pl.emul_mask & EMUL_MONOPOLE) {
+            double qc = 0;
+            for (int b = c.boff; b < c.boff + c.bnum; b++) qc += q[b];
+            set_monopole(&pl.qeq[(size_t)c.idx * S], S, qc);
+        }
     }
 }
 
@@ -706,8 +808,13 @@ static void stage_m2m(Plan& pl, int mind, int maxd, const double* qtot = nullptr
             if (d >= 0) {
                 for (int ch = 0; ch < c.cnum; ch++) {
                     const Cell& cc = pl.cells[c.coff + ch];
-                    gemv(pl.m2m[cc.octant].data(), &pl.qeq[(size_t)cc.idx * S], tmp.data(), S, S, 1.0);
+                    gemv_stage(pl, EMUL_M2M, pl.m2m[cc.octant].data(), S, &pl.qeq[(size_t)cc.idx * S], tmp.data(), S, S, 1.0);
                     for (int s = 0; s < S; s++) qp[s] += tmp[s];
+                }
+                if (pl.emul_mask & EMUL_MONOPOLE) {
+                    double qcell = 0;
+                    for (int b = c.boff; b < c.boff + c.bnum; b++) qcell += pl.q[b];
+                    set_monopole(qp, S, qcell);
                 }
             } else {
                 const double* qc = &pl.qeq[(size_t)c.coff * S];
@@ -869,7 +976,7 @@ static void stage_l2l(Plan& pl, int mind, int maxd) {
             if (d >= 0) {
                 for (int ch = 0; ch < c.cnum; ch++) {
                     const Cell& cc = pl.cells[c.coff + ch];
-                    gemv(pl.l2l[cc.octant].data(), pp, tmp.data(), S, S, 1.0);
+                    gemv_stage(pl, EMUL_L2L, pl.l2l[cc.octant].data(), S, pp, tmp.data(), S, S, 1.0);
                     double* pc = &pl.pck[(size_t)cc.idx * S];
                     for (int s = 0; s < S; s++) pc[s] += tmp[s];
                 }
@@ -891,8 +998,28 @@ static void stage_l2p(Plan& pl, const std::vector<int>& leaves, double* p, doubl
         std::vector<double> eq, buf(S), qd(S);
         surface_points(pl.P, c.r * 2.95, c.x, eq);
         double scale = std::pow(0.5, c.depth);
-        gemv(pl.UT_l2p.data(), &pl.pck[(size_t)c.idx * S], buf.data(), S, S, 1.0);
-        gemv(pl.VSinv_l2p.data(), buf.data(), qd.data(), S, S, scale);
+        const int r = pl.pinv_rank > 0 ? std::min(pl.pinv_rank, S) : S;
+        gemv_stage(pl, EMUL_L2P_PINV, pl.UT_l2p.data(), S, &pl.pck[(size_t)c.idx * S], buf.data(), r, S, 1.0);
+        gemv_stage(pl, EMUL_L2P_PINV, pl.VSinv_l2p.data(), S, buf.data(), qd.data(), S, r, scale);
+        if (pl.emul_mask & EMUL_L2P_EVAL) {   // float32 cell-relative differences, products and sums
+            for (int b = c.boff; b < c.boff + c.bnum; b++) {
+                float ap = 0, ax = 0, ay = 0, az = 0;
+                const float bxr = (float)(pl.bx[b] - c.x[0]), byr = (float)(pl.by[b] - c.x[1]),
+                            bzr = (float)(pl.bz[b] - c.x[2]);
+                for (int s = 0; s < S; s++) {
+                    float dx = bxr - (float)(eq[3 * s] - c.x[0]), dy = byr - (float)(eq[3 * s + 1] - c.x[1]),
+                          dz = bzr - (float)(eq[3 * s + 2] - c.x[2]);
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    float ir = r2 > 0 ? 1.0f / std::sqrt(r2) : 0.0f;
+                    float qi = (float)qd[s] * ir;
+                    ap += qi;
+                    qi *= ir * ir;
+                    ax += qi * dx; ay += qi * dy; az += qi * dz;
+                }
+                p[b] += ap; gx[b] -= ax; gy[b] -= ay; gz[b] -= az;
+            }
+            continue;
+        }
         for (int b = c.boff; b < c.boff + c.bnum; b++) {
             double ap = 0, ax = 0, ay = 0, az = 0;
             for (int s = 0; s < S; s++) {
@@ -986,6 +1113,9 @@ int orc_set_option(void* h, const char* name, int v) {
     Plan* pl = (Plan*)h;
     if (!strcmp(name, "m2l_bruteforce")) { pl->m2l_bruteforce = v; return 0; }
     if (!strcmp(name, "threads")) { pl->nthreads = v; return 0; }
+    if (!strcmp(name, "emulate")) { pl->emul_mask = v; return 0; }
+    if (!strcmp(name, "emulate_kind")) { pl->emul_kind = v; return v >= EMUL_F32 && v <= EMUL_BF16X6 ? 0 : -1; }
+    if (!strcmp(name, "pinv_rank")) { pl->pinv_rank = v; return 0; }
     return -1;
 }
 

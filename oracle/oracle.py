@@ -105,6 +105,17 @@ class OracleFMM:
         lib().orc_precompute(self.h)
         self._ops = True
 
+    # Stage bits and arithmetic of the reduced-precision emulation (orc_set_option "emulate").
+    EMUL_P2M_CHECK, EMUL_P2M_PINV, EMUL_M2M, EMUL_L2L, EMUL_L2P_PINV, EMUL_L2P_EVAL, EMUL_MONOPOLE = 1, 2, 4, 8, 16, 32, 64
+    EMUL_F32, EMUL_BF16X3, EMUL_BF16X6 = 0, 1, 2
+
+    def set_emulation(self, mask, kind=0, pinv_rank=0):
+        """Run the stages in `mask` in the emulated arithmetic `kind` (float32, or split bfloat16
+        products accumulated in float32), and the P2M/L2P pseudo-inverses at `pinv_rank` (0 = full)."""
+        for name, v in ((b"emulate", mask), (b"emulate_kind", kind), (b"pinv_rank", pinv_rank)):
+            if lib().orc_set_option(self.h, name, int(v)) != 0:
+                raise ValueError(f"bad emulation option {name.decode()}={v}")
+
     def set_particles(self, x, y, z):
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
         self.n = len(x)

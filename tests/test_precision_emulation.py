@@ -1,0 +1,89 @@
+"""Which operator applies of the mixed-precision path may run in float32: the restated reference with
+each operator stage in an emulated arithmetic, against its own float64 result (infinite-lattice
+gauge, P=8, the pseudo-inverses at the mixed path's truncated rank 192).
+
+Stages: P2M check potentials, P2M pseudo-inverse, M2M, L2L, L2P pseudo-inverse, L2P evaluation.
+Arithmetics: float32 (operands and sums), split bfloat16 with 2 pieces and 3 products ("BF16x3")
+and with 3 pieces and 6 products, each product exact and summed in float32.  Findings pinned here:
+
+* float32 everywhere leaves the gradient at the truncated rank's own level (~1e-7) but shifts the
+  potential by a constant of a few 1e-6: the equivalent densities' monopoles pick up float32
+  rounding, and every cell's M2L turns its monopole error into a potential offset;
+* constraining each tree cell's equivalent-density monopole to the cell's charge (uniform shift of
+  the density after P2M and M2M) removes the offset: float32 in all six stages stays below 1e-6;
+* BF16x3 under the same constraint is an order of magnitude worse than float32 (~1e-5 in the
+  potential, which is the whole error budget), so the matrix-core operator applies use float32.
+"""
+import numpy as np
+import pytest
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _relc(a, b):
+    d = a - b
+    return float(np.linalg.norm(d - d.mean()) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def emul(oracle_mod):
+    rng = np.random.Generator(np.random.PCG64(42))
+    n = 32768
+    pts = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    f = oracle_mod.OracleFMM(8, 64, 2, 1.0, dipole=2, threads=8)
+    f.precompute()
+    f.set_particles(pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy())
+    p0, g0 = f.evaluate(q)
+
+    def run(mask, kind=0, rank=192):
+        f.set_emulation(mask, kind, rank)
+        p, g = f.evaluate(q)
+        return _rel(p, p0), _relc(p, p0), _rel(g, g0)
+
+    yield f, run, (p0, g0, q)
+    f.set_emulation(0, 0, 0)
+
+
+def test_emulation_off_is_the_plain_path(emul):
+    f, run, (p0, g0, q) = emul
+    f.set_emulation(0, 0, 0)
+    p, g = f.evaluate(q)
+    assert np.array_equal(p, p0) and np.array_equal(g, g0)
+
+
+def test_float32_upward_pass_shifts_the_potential_by_a_constant(emul):
+    f, run, _ = emul
+    O = type(f)
+    up = O.EMUL_P2M_PINV | O.EMUL_M2M
+    raw, modc, grad = run(up)
+    print(f"float32 P2M pinv + M2M, no monopole constraint: phi {raw:.2e} (mod const {modc:.2e}) grad {grad:.2e}")
+    assert raw > 1e-6 and raw > 3 * modc and grad < 3e-7
+
+
+def test_float32_everywhere_with_the_monopole_constraint(emul):
+    f, run, _ = emul
+    O = type(f)
+    base = run(0)
+    allx = (O.EMUL_P2M_CHECK | O.EMUL_P2M_PINV | O.EMUL_M2M | O.EMUL_L2L | O.EMUL_L2P_PINV | O.EMUL_L2P_EVAL
+            | O.EMUL_MONOPOLE)
+    raw, modc, grad = run(allx)
+    print(f"truncated rank only: phi {base[0]:.2e} grad {base[2]:.2e}; float32 in all six stages + monopole "
+          f"constraint: phi {raw:.2e} (mod const {modc:.2e}) grad {grad:.2e}")
+    assert raw < 1e-6 and grad < 3e-7
+
+
+def test_bf16x3_is_much_worse_than_float32(emul):
+    f, run, _ = emul
+    O = type(f)
+    ops = O.EMUL_P2M_PINV | O.EMUL_M2M | O.EMUL_L2L | O.EMUL_L2P_PINV | O.EMUL_MONOPOLE
+    f32 = run(ops, O.EMUL_F32)
+    b3 = run(ops, O.EMUL_BF16X3)
+    b6 = run(ops, O.EMUL_BF16X6)
+    print(f"operator GEMMs + monopole constraint: float32 phi {f32[0]:.2e} grad {f32[2]:.2e}; "
+          f"bf16x3 phi {b3[0]:.2e} grad {b3[2]:.2e}; bf16 3 pieces/6 products phi {b6[0]:.2e} grad {b6[2]:.2e}")
+    assert b3[0] > 5 * f32[0] and b3[0] > 2e-6
+    assert b6[0] < 2 * f32[0] + 2e-7
