@@ -263,6 +263,7 @@ int fmm3dpp_stage_p2p(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells, 
         std::vector<int32_t> se;
         for (int64_t i = 0; i < ncells; i++) {
             const int j = leaf_of[cells[i]];
+            if (j < 0) FAIL(FMM3DPP_ERR_ARG, "listed cell is not in the leaf list");
             se.insert(se.end(), plan->h_p2p_ent.begin() + plan->h_p2p_off[j], plan->h_p2p_ent.begin() + plan->h_p2p_off[j + 1]);
             so.push_back((int64_t)se.size());
         }
