@@ -57,6 +57,7 @@ struct Group {
     M2LTiles up_tiles, down_tiles;   // transform tiles of the tree parents (0, 0: measured defaults)
     // work buffers
     DBuf q, pot, gx, gy, gz, pcleaf, pcleaf2, qeq, pck, Qk, Pk, Qk_img, Pk_img, rel, m2m_part, partial, moments;
+    DBuf qcell;   // per-cell charges of the mixed path's monopole constraint
 };
 
 }  // namespace
@@ -121,7 +122,7 @@ double now_ms() {
 }
 
 int upload_group_operators(fmm3dpp_batch* b, Group& g) {
-    BHIP(g.dop.upload(g.ops, b->mixed(), b->stream));
+    BHIP(g.dop.upload(g.ops, b->mixed(), true, b->stream));
     g.ops_uploaded = true;
     return FMM3DPP_OK;
 }
@@ -346,7 +347,7 @@ int build_group(fmm3dpp_batch* b, Group& g) {
     BHIP(g.q.ensure(nb * 8)); BHIP(g.pot.ensure(nb * 8));
     BHIP(g.gx.ensure(nb * 8)); BHIP(g.gy.ensure(nb * 8)); BHIP(g.gz.ensure(nb * 8));
     BHIP(g.pcleaf.ensure(nlf * Sn * 8)); BHIP(g.pcleaf2.ensure(nlf * Sn * 8));
-    BHIP(g.qeq.ensure(nc * Sn * 8)); BHIP(g.pck.ensure(nc * Sn * 8));
+    BHIP(g.qeq.ensure(nc * Sn * 8)); BHIP(g.pck.ensure(nc * Sn * 8)); BHIP(g.qcell.ensure(nc * 8));
     BHIP(launch_fill_zero(g.qeq.as<double>(), (int64_t)nc * Sn, s));
     if (b->mixed()) {
         BHIP(g.Qk.ensure(F * std::max(nl, 1) * 64)); BHIP(g.Pk.ensure(F * std::max(nl, 1) * 64));
@@ -367,7 +368,6 @@ int build_group(fmm3dpp_batch* b, Group& g) {
 int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, double* d_grad, hipStream_t s) {
     const Operators& o = g.ops;
     const int Sn = o.S, nleaf = g.nleaf;
-    const size_t SS = (size_t)Sn * Sn;
     const CellsDev cells{g.cx.as<double>(), g.cy.as<double>(), g.cz.as<double>(), g.cr.as<double>(),
                          g.depth.as<int32_t>(), g.boff.as<int32_t>(), g.bnum.as<int32_t>()};
     const BodiesDev bod{g.x.as<double>(), g.y.as<double>(), g.z.as<double>(), g.q.as<double>()};
@@ -375,27 +375,31 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     double* pck = g.pck.as<double>();
     const int32_t* iota = g.iota.as<int32_t>();
     const bool mixed = b->mixed();
+    const bool f32 = mixed && f32_apply_orders(o.P);   // float32 tree operator stages
     BHIP(launch_gather(d_qin, g.perm.as<int32_t>(), g.q.as<double>(), (int)g.n, s));
     BHIP(launch_fill_zero(pck, (int64_t)g.ncells * Sn, s));
     // P2M
     BHIP(launch_p2m_check(Sn, g.dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod,
-                          g.pcleaf.as<double>(), s));
+                          g.pcleaf.as<double>(), s, f32));
     {
-        GemmCols g1{Sn, nleaf, 1, 0, g.dop.o_ut.as<double>(), 0, g.pcleaf.as<double>(), iota, iota, 0,
-                    g.pcleaf2.as<double>(), 0, nullptr};
-        g1.M = g.dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-        BHIP(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, g.dop.o_vs.as<double>(), 0, g.pcleaf2.as<double>(), iota, g.leaf_cell.as<int32_t>(), 0,
-                    qeq, 0, g.leaf_depth.as<int32_t>()};
-        g2.K = g.dop.pinv_rank;
-        BHIP(launch_gemm_cols(g2, s));
+        const ApplyCols g1{Sn, nleaf, 1, false, g.pcleaf.as<double>(), iota, iota, 0, g.pcleaf2.as<double>(), 0, nullptr};
+        BHIP(launch_apply(g1, g.dop.lo_ut, s));
+        const ApplyCols g2{Sn, nleaf, 1, false, g.pcleaf2.as<double>(), iota, g.leaf_cell.as<int32_t>(), 0, qeq, 0,
+                           g.leaf_depth.as<int32_t>()};
+        BHIP(launch_apply(g2, g.dop.lo_vs, s));
+        if (f32)
+            BHIP(launch_monopole_leaves(qeq, Sn, g.leaf_cell.as<int32_t>(), nleaf, cells, g.q.as<double>(),
+                                        g.qcell.as<double>(), s));
     }
     // M2M per level, then the image levels (level m of every system in one launch)
     for (const auto& lv : g.m2m_levels) {
-        GemmCols gm{Sn, lv.ncol, 8, 0, g.dop.o_m2m_all.as<double>(), (int64_t)SS, qeq, g.m2m_in.as<int32_t>() + lv.in_off,
-                    iota, lv.ncol, g.m2m_part.as<double>(), 0, nullptr};
-        BHIP(launch_gemm_cols(gm, s));
+        const ApplyCols gm{Sn, lv.ncol, 8, false, qeq, g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol,
+                           g.m2m_part.as<double>(), 0, nullptr};
+        BHIP(launch_apply(gm, g.dop.lo_m2m, s));
         BHIP(launch_sum_partials(Sn, lv.ncol, 8, g.m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off, qeq, s));
+        if (f32)
+            BHIP(launch_monopole_parents(qeq, Sn, g.m2m_out.as<int32_t>() + lv.out_off, lv.ncol,
+                                         g.m2m_in.as<int32_t>() + lv.in_off, g.qcell.as<double>(), s));
     }
     // infinite-lattice gauge: every system's root and image cells get their charge as monopole (the
     // system's total charge times 27^m for c(-m)); the per-system moments are those of the finalize
@@ -408,9 +412,9 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     }
     double img_scale = 1.0;
     for (const auto& lv : g.img_m2m_lv) {
-        GemmCols gi{Sn, (int)lv[1], 1, 0, g.dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + lv[0],
-                    g.img_a.as<int32_t>() + lv[0], 0, qeq, 0, nullptr};
-        BHIP(launch_gemm_cols(gi, s));
+        const ApplyCols gi{Sn, (int)lv[1], 1, false, qeq, g.img_b.as<int32_t>() + lv[0], g.img_a.as<int32_t>() + lv[0], 0,
+                           qeq, 0, nullptr};
+        BHIP(launch_apply(gi, g.dop.lo_m2m_img, s));
         img_scale *= 27.0;
         if (neutral)
             BHIP(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + lv[0], 0, (int)lv[1], g.moments.as<double>() + 4, 5,
@@ -445,28 +449,25 @@ int run_group(fmm3dpp_batch* b, Group& g, const double* d_qin, double* d_phi, do
     }
     // L2L: image levels from the outermost, then tree levels
     for (const auto& lv : g.img_l2l_lv) {
-        GemmCols gi{Sn, (int)lv[1], 1, 1, g.dop.o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + lv[0],
-                    g.img_b.as<int32_t>() + lv[0], 0, pck, 0, nullptr};
-        BHIP(launch_gemm_cols(gi, s));
+        const ApplyCols gi{Sn, (int)lv[1], 1, true, pck, g.img_a.as<int32_t>() + lv[0], g.img_b.as<int32_t>() + lv[0], 0,
+                           pck, 0, nullptr};
+        BHIP(launch_apply(gi, g.dop.lo_l2l_img, s));
     }
     for (const auto& lv : g.l2l_levels) {
-        GemmCols gl{Sn, lv.ncol, 8, 1, g.dop.o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
-                    g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
-        BHIP(launch_gemm_cols(gl, s));
+        const ApplyCols gl{Sn, lv.ncol, 8, true, pck, g.l2l_in.as<int32_t>() + lv.in_off,
+                           g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
+        BHIP(launch_apply(gl, g.dop.lo_l2l, s));
     }
     // L2P
     {
-        GemmCols g1{Sn, nleaf, 1, 0, g.dop.o_utl.as<double>(), 0, pck, g.leaf_cell.as<int32_t>(), iota, 0,
-                    g.pcleaf2.as<double>(), 0, nullptr};
-        g1.M = g.dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-        BHIP(launch_gemm_cols(g1, s));
-        GemmCols g2{Sn, nleaf, 1, 0, g.dop.o_vsl.as<double>(), 0, g.pcleaf2.as<double>(), iota, iota, 0,
-                    g.pcleaf.as<double>(), 0, g.leaf_depth.as<int32_t>()};
-        g2.K = g.dop.pinv_rank;
-        BHIP(launch_gemm_cols(g2, s));
+        const ApplyCols g1{Sn, nleaf, 1, false, pck, g.leaf_cell.as<int32_t>(), iota, 0, g.pcleaf2.as<double>(), 0, nullptr};
+        BHIP(launch_apply(g1, g.dop.lo_utl, s));
+        const ApplyCols g2{Sn, nleaf, 1, false, g.pcleaf2.as<double>(), iota, iota, 0, g.pcleaf.as<double>(), 0,
+                           g.leaf_depth.as<int32_t>()};
+        BHIP(launch_apply(g2, g.dop.lo_vsl, s));
     }
     BHIP(launch_l2p_eval(Sn, g.dop.o_usurf.as<double>(), g.leaf_cell.as<int32_t>(), nleaf, cells, bod, g.pcleaf.as<double>(),
-                         g.pot.as<double>(), g.gx.as<double>(), g.gy.as<double>(), g.gz.as<double>(), s));
+                         g.pot.as<double>(), g.gx.as<double>(), g.gy.as<double>(), g.gz.as<double>(), s, f32));
     // P2P
     const double cyc = b->cfg[0].cycle;
     if (!mixed) {

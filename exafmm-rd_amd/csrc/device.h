@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "dbuf.h"
 
@@ -30,30 +31,56 @@ struct BodiesDev {
 inline bool valid(const CellsDev& c) { return c.x && c.y && c.z && c.r && c.depth && c.boff && c.bnum; }
 inline bool valid(const BodiesDev& b) { return b.x && b.y && b.z && b.q; }
 
-// Column-gathered batched GEMM, batch z = blockIdx.z:
-//   Y_z[out_idx[z*idx_zstride + c]][0:S] (+)= scale_c * A_z * X[in_idx[z*idx_zstride + c]][0:S]
-// with A_z = A + z*a_zstride (S x S row-major), X and Y_z = Y + z*y_zstride laid out [vector][S];
-// in_idx < 0 reads zeros, out_idx < 0 skips the column; scale_c = 0.5^scale_depth[c] if given.
-struct GemmCols {
-    int S, ncol, nz, accumulate;
-    const double* A;
-    int64_t a_zstride;
+// Columns of an operator apply, batch z = blockIdx.z:
+//   Y_z[out_idx[z*idx_zstride + c]][0:M] (+)= scale_c * A_z * X[in_idx[z*idx_zstride + c]][0:K]
+// with X and Y_z = Y + z*y_zstride laid out [vector][S]; in_idx < 0 reads zeros, out_idx < 0 skips
+// the column; scale_c = 0.5^scale_depth[c] if given.  The operator A_z (M x K) is a LaneOp.
+struct ApplyCols {
+    int S, ncol, nz;
+    bool accumulate;
     const double* X;
     const int32_t* in_idx;
     const int32_t* out_idx;
-    int32_t idx_zstride;
+    int64_t idx_zstride;
     double* Y;
     int64_t y_zstride;
     const int32_t* scale_depth;
-    // Rows of A used (M) and the inner dimension (K), 0 for S: a leading block of the S x S operator
-    // (row stride S), as the truncated pseudo-inverse applies use.  Vectors keep stride S.
-    int M = 0, K = 0;
 };
 
+// An operator, or a batch of nz operators, in the lane layout of the apply kernels: element
+// [kc][b][u][lane] = A[16 b + lane % 16][16 kc + 4 u + lane / 16] (zero outside M x K), operator z at
+// p + z * zstride elements; float64, or float32 for the float32 matrix cores.
+struct LaneOp {
+    const void* p = nullptr;
+    bool f32 = false;
+    int M = 0, K = 0;       // rows and inner dimension
+    int nmb = 0, nkc = 0;   // 16-row blocks and 16-k chunks: (M + 15) / 16, (K + 15) / 16
+    int64_t zstride = 0;    // nmb * nkc * 256
+    int nops = 0;           // operators held (a launch's batch may use up to this many)
+};
+
+// Y (+)= A X over the columns (kernels_apply.hip).  mbw > 0 forces the tiled kernel with that many
+// 16-row blocks per workgroup (20, 12, 8 or 4); the result is the same bits for every choice.
+hipError_t launch_apply(const ApplyCols& g, const LaneOp& op, hipStream_t st, int mbw = 0);
+// Once per process before the first apply (and before any graph capture): LDS limits of the kernels.
+hipError_t apply_init();
+// out[at ...] = the lane layout of the M x K leading block of A (row stride lda).
+template <class T>
+void lane_layout(const double* A, int lda, int M, int K, std::vector<T>& out, size_t at);
+// Per-cell monopole constraint of the mixed path: each listed cell's equivalent density is shifted
+// uniformly so that its sum equals the cell's charge, recorded in qcell.  Leaves: the charge of the
+// leaf's bodies (tree order).  Parents of a level: the sum of the children's recorded charges, in
+// octant order over the level's child table children[o * ncol + col] (-1: no child).
+hipError_t launch_monopole_leaves(double* qeq, int S, const int32_t* leaf_cell, int nleaf, CellsDev cells,
+                                  const double* q, double* qcell, hipStream_t st);
+hipError_t launch_monopole_parents(double* qeq, int S, const int32_t* parents, int ncol, const int32_t* children,
+                                   double* qcell, hipStream_t st);
+
 hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int n, hipStream_t st);
+// f32: float32 form (coordinates relative to the leaf centre, float32 sums) of the mixed path's
+// float32 operator applies.
 hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
-                            BodiesDev b, double* pc, hipStream_t st);
-hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st);
+                            BodiesDev b, double* pc, hipStream_t st, bool f32 = false);
 // Y[out[c]][r] = sum_{z < nz} part[z][c][r] for out[c] >= 0 (fixed order: deterministic M2M child sum)
 hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
                                hipStream_t st);
@@ -95,7 +122,7 @@ hipError_t launch_m2l_down(int P, bool f32, int S, int ntar, const int32_t* tar_
 bool m2l_tiles_valid(int P, bool f32, bool down, M2LTiles tiles);
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
-                           hipStream_t st);
+                           hipStream_t st, bool f32 = false);
 // list_ent[e] = source leaf cell << 5 | periodic offset index (offset = ((i%3)-1, (i/3%3)-1, (i/9)-1) * cycle).
 // accumulate: p += phi_near, g -= sum q d / r^3 (onto the far field); otherwise the near field alone
 // is stored (p = phi_near, g = -sum q d / r^3), for a finalize that adds the two.

@@ -1,6 +1,6 @@
 // Float64 kernels shared by both precision paths of the periodic KIFMM on CDNA4 (gfx950): charge
-// gather, the column-gathered operator GEMM (pseudo-inverse applies, M2M, L2L), the M2M child sum,
-// the dipole moment and the final correction/reorder.
+// gather, the M2M child sum, the image-cell monopoles, the dipole moment and the final
+// correction/reorder.  The operator applies are in kernels_apply.hip.
 //
 // Data layout in HBM:
 //   bodies      SoA x, y, z, q, p, gx, gy, gz in tree order (leaf bodies contiguous)
@@ -21,138 +21,6 @@ constexpr int kMoments = 5;   // sum q x, sum q y, sum q z, sum q |x|^2, sum q
 __global__ void k_gather(const double* __restrict__ qin, const int32_t* __restrict__ perm, double* __restrict__ qt, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) qt[i] = qin[perm[i]];
-}
-
-// Column-gathered batched GEMM for the operator applies: P2M/L2P pseudo-inverse factors
-// (src/kernel.cpp:369-370, 1465-1467), M2M per child octant (395-404), L2L per child octant
-// (1419-1428) and the image-level M2M/L2L (434-442, 1430-1435), on the float64 matrix cores.
-// Computed transposed, D[v][r] = sum_k X[v][k] A[r][k] (v = gathered vector, r = output
-// component), so that a lane's accumulator rows are vectors and its columns consecutive
-// components: stores of 16 lanes cover 128 contiguous bytes of one output vector.
-// Workgroup tile: 64 vectors x 64 components; wave tile 32 x 32 = 2 x 2 blocks of
-// v_mfma_f64_16x16x4f64 (lane l: A operand [l % 16][l / 16], B operand [l / 16][l % 16],
-// accumulator register i = D[l / 16 + 4 i][l % 16]).  K in LDS stages of 16 with the next stage
-// prefetched into registers.
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-constexpr int TM = 64, TN = 64, TK = 16;
-constexpr int kGemmSmallCols = 128;   // column counts up to this go to k_gemm_cols_small
-__global__ __launch_bounds__(256) void k_gemm_cols(GemmCols g) {
-    __shared__ double As[TK][TM + 4];   // operator rows r (components), k-major
-    __shared__ double Bs[TK][TN + 4];   // gathered vectors v, k-major
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row0 = blockIdx.x * TM, col0 = blockIdx.y * TN, z = blockIdx.z;
-    const int S = g.S, M = g.M ? g.M : S, K = g.K ? g.K : S;
-    const double* A = g.A + (int64_t)z * g.a_zstride;
-    const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
-    const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
-    double* Y = g.Y + (int64_t)z * g.y_zstride;
-    const int lr = tid >> 2, lk = (tid & 3) * 4;
-    const int gr = row0 + lr, gc = col0 + lr;
-    const int src = gc < g.ncol ? in_idx[gc] : -1;
-    const double* arow = gr < M ? A + (int64_t)gr * S : nullptr;
-    const double* bcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
-    double ra[4], rb[4];
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            int gk = k0 + lk + u;
-            ra[u] = (arow && gk < K) ? arow[gk] : 0.0;
-            rb[u] = (bcol && gk < K) ? bcol[gk] : 0.0;
-        }
-    };
-    const int n = lane & 15, kq = lane >> 4;
-    const int vb0 = 2 * (wave & 1), rb0 = 2 * (wave >> 1);   // this wave's 16-blocks of v and r
-    f64x4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f64x4{0, 0, 0, 0};
-    fetch(0);
-    for (int k0 = 0; k0 < K; k0 += TK) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) { As[lk + u][lr] = ra[u]; Bs[lk + u][lr] = rb[u]; }
-        __syncthreads();
-        if (k0 + TK < K) fetch(k0 + TK);
-#pragma unroll
-        for (int k4 = 0; k4 < TK / 4; k4++) {
-            const int kk = 4 * k4 + kq;
-            double xv[2], av[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++) xv[i] = Bs[kk][(vb0 + i) * 16 + n];   // A operand: X[v][k]
-#pragma unroll
-            for (int j = 0; j < 2; j++) av[j] = As[kk][(rb0 + j) * 16 + n];   // B operand: A[r][k]
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i], av[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = col0 + (vb0 + i) * 16 + kq + 4 * q;
-            if (c >= g.ncol) continue;
-            const int oi = out_idx[c];
-            if (oi < 0) continue;
-            const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int r = row0 + (rb0 + j) * 16 + n;
-                if (r >= M) continue;
-                const double v = sc * acc[i][j][q];
-                double* y = &Y[(int64_t)oi * S + r];
-                *y = g.accumulate ? *y + v : v;
-            }
-        }
-}
-
-// Few columns (the upper tree levels and the image levels): one wave per 16 x 16 output block,
-// operands read straight from L2 with eight K steps of loads in flight together, instead of one
-// LDS stage of 16 k behind a barrier per step (a 64-column level took ~22 us, all latency).  Every
-// output element sees the same MFMA sequence as in k_gemm_cols (K in steps of 4 from k = 0, the
-// same operand lanes; k_gemm_cols's zero-padded steps past K add exact zeros), so the results are
-// the same bits.
-__global__ __launch_bounds__(64) void k_gemm_cols_small(GemmCols g) {
-    const int lane = threadIdx.x, n = lane & 15, kq = lane >> 4;
-    const int rb = blockIdx.x, vb = blockIdx.y, z = blockIdx.z;   // 16 components x 16 vectors
-    const int S = g.S, M = g.M ? g.M : S, K = g.K ? g.K : S;
-    const double* A = g.A + (int64_t)z * g.a_zstride;
-    const int32_t* in_idx = g.in_idx + (int64_t)z * g.idx_zstride;
-    const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
-    double* Y = g.Y + (int64_t)z * g.y_zstride;
-    const int r = rb * 16 + n, cv = vb * 16 + n;   // the component and the vector this lane feeds
-    const int src = cv < g.ncol ? in_idx[cv] : -1;
-    const double* arow = r < M ? A + (int64_t)r * S : nullptr;
-    const double* xcol = src >= 0 ? g.X + (int64_t)src * S : nullptr;
-    f64x4 acc = {0, 0, 0, 0};
-    const int nk4 = (K + 3) / 4;
-    constexpr int U = 8;
-    for (int k40 = 0; k40 < nk4; k40 += U) {
-        double xv[U], av[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = 4 * (k40 + u) + kq;
-            xv[u] = (xcol && k < K) ? xcol[k] : 0.0;
-            av[u] = (arow && k < K) ? arow[k] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k40 + u < nk4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[u], av[u], acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int c = vb * 16 + kq + 4 * q;
-        if (c >= g.ncol || r >= M) continue;
-        const int oi = out_idx[c];
-        if (oi < 0) continue;
-        const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
-        const double v = sc * acc[q];
-        double* y = &Y[(int64_t)oi * S + r];
-        *y = g.accumulate ? *y + v : v;
-    }
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(int S, int ncol, int nz, const double* __restrict__ part,
@@ -212,15 +80,16 @@ __global__ __launch_bounds__(256) void k_dipole_partial(BodiesDev b, int n, doub
 }
 
 
-// Moments of each segment (a system; the whole body set for a plan) in a fixed order: the sum of the
-// segment's per-block partials, block by block.
-__global__ __launch_bounds__(64) void k_moments_seg(const double* __restrict__ partial, int nblocks,
-                                                    double* __restrict__ moments) {
-    const int seg = blockIdx.x, k = threadIdx.x;
-    if (k >= kMoments) return;
+// Moments of each segment (a system; the whole body set for a plan) in a fixed order: wave k sums
+// moment k of the segment's per-block partials, lane l the blocks l, l + 64, ..., then an xor
+// butterfly over the lanes (one chain of nblocks dependent loads per moment took 31 us).
+__global__ __launch_bounds__(64 * kMoments) void k_moments_seg(const double* __restrict__ partial, int nblocks,
+                                                               double* __restrict__ moments) {
+    const int seg = blockIdx.x, k = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double a = 0;
-    for (int j = 0; j < nblocks; j++) a += partial[((size_t)seg * nblocks + j) * kMoments + k];
-    moments[seg * kMoments + k] = a;
+    for (int j = lane; j < nblocks; j += 64) a += partial[((size_t)seg * nblocks + j) * kMoments + k];
+    for (int m = 32; m > 0; m >>= 1) a += __shfl_xor(a, m);
+    if (lane == 0) moments[seg * kMoments + k] = a;
 }
 
 // Dipole correction, 1/(4 pi) scaling (src/body.cpp:88-96) and the return to input order
@@ -364,21 +233,6 @@ hipError_t launch_gather(const double* qin, const int32_t* perm, double* qt, int
     return hipGetLastError();
 }
 
-hipError_t launch_gemm_cols(const GemmCols& g, hipStream_t st) {
-    if (g.ncol <= 0) return hipSuccess;
-    if (!g.A || !g.X || !g.Y || !g.in_idx || !g.out_idx || g.S <= 0 || g.nz <= 0) return hipErrorInvalidValue;
-    if (g.M < 0 || g.M > g.S || g.K < 0 || g.K > g.S) return hipErrorInvalidValue;
-    const int M = g.M ? g.M : g.S;
-    if (g.ncol <= kGemmSmallCols) {   // same bits as the tiled kernel
-        dim3 grid((M + 15) / 16, (g.ncol + 15) / 16, g.nz);
-        k_gemm_cols_small<<<grid, 64, 0, st>>>(g);
-        return hipGetLastError();
-    }
-    dim3 grid((M + TM - 1) / TM, (g.ncol + TN - 1) / TN, g.nz);
-    k_gemm_cols<<<grid, 256, 0, st>>>(g);
-    return hipGetLastError();
-}
-
 hipError_t launch_sum_partials(int S, int ncol, int nz, const double* part, const int32_t* out, double* Y,
                                hipStream_t st) {
     if (ncol <= 0) return hipSuccess;
@@ -398,7 +252,7 @@ hipError_t launch_neutralize(double* qeq, int S, const int32_t* cells, int first
 
 hipError_t launch_moments(const double* partial, int nblocks, double* moments, hipStream_t st) {
     if (!partial || !moments || nblocks <= 0) return hipErrorInvalidValue;
-    k_moments_seg<<<1, 64, 0, st>>>(partial, nblocks, moments);
+    k_moments_seg<<<1, 64 * kMoments, 0, st>>>(partial, nblocks, moments);
     return hipGetLastError();
 }
 
@@ -416,7 +270,7 @@ hipError_t launch_finalize(int n, const int32_t* perm, BodiesDev b, const double
         return hipErrorInvalidValue;
     if (near.p && (!near.gx || !near.gy || !near.gz)) return hipErrorInvalidValue;
     double* M = partial + (size_t)nblocks * kMoments;   // row nblocks: the summed moments
-    k_moments_seg<<<1, 64, 0, st>>>(partial, nblocks, M);
+    k_moments_seg<<<1, 64 * kMoments, 0, st>>>(partial, nblocks, M);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_finalize<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, p, gx, gy, gz, near, M, mode, cycle, out_scale, phi,
@@ -433,7 +287,7 @@ hipError_t launch_adjoint_combine(int n, const int32_t* perm, BodiesDev b, doubl
         nblocks <= 0)
         return hipErrorInvalidValue;
     double* Wm = partial_w + (size_t)nblocks * kMoments;   // row nblocks: the summed moments of w
-    k_moments_seg<<<1, 64, 0, st>>>(partial_w, nblocks, Wm);
+    k_moments_seg<<<1, 64 * kMoments, 0, st>>>(partial_w, nblocks, Wm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_adjoint_combine<<<(n + 255) / 256, 256, 0, st>>>(n, perm, b, Wm, c, out_scale, qin, win, grad_phi, psi,
@@ -448,7 +302,7 @@ hipError_t launch_dipole_seg(BodiesDev b, const int64_t* seg_off, int nseg, int 
     k_dipole_seg<<<dim3(nblocks, nseg), 256, 0, st>>>(b, seg_off, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_moments_seg<<<nseg, 64, 0, st>>>(partial, nblocks, moments);
+    k_moments_seg<<<nseg, 64 * kMoments, 0, st>>>(partial, nblocks, moments);
     return hipGetLastError();
 }
 

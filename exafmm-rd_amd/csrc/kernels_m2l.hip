@@ -406,9 +406,11 @@ __global__ __launch_bounds__(256) void k_hadamard27(int F, int ntab, int base,
 // p_check (float64).  One workgroup per target parent and group of SG child slots; the half
 // spectrum is taken FC columns fi at a time: z pass (N values of Pk[t][f][slot] -> P outputs),
 // y pass (N -> P), and the Hermitian x pass accumulates into per-thread registers, so only the two
-// column chunks live in LDS.
+// column chunks live in LDS.  blockIdx.x is remapped so that the slot groups of a parent run on one XCD
+// at about the same time: each reads part of every 64-byte (float32) Pk vector, and the line one
+// fetches from HBM is then still in that XCD's L2 for the others.
 template <int P, typename T>
-__global__ __launch_bounds__(256) void k_m2l_down(int S, int SG, int FC, const int32_t* __restrict__ tar_child,
+__global__ __launch_bounds__(256) void k_m2l_down(int S, int SG, int FC, int nblk, const int32_t* __restrict__ tar_child,
                                                   const double* __restrict__ tar_scale,
                                                   const int32_t* __restrict__ inv_lat,
                                                   const double* __restrict__ twiddle, const cplx<T>* __restrict__ Pk,
@@ -417,7 +419,10 @@ __global__ __launch_bounds__(256) void k_m2l_down(int S, int SG, int FC, const i
     constexpr int N = 2 * P - 1, Nh = N / 2 + 1, N3 = N * N * N, F = N * N * Nh;
     constexpr int XI = (8 * P * P + 255) / 256;                  // x-pass items per thread (SG <= 8)
     const int ngrp = 8 / SG;
-    const int t = blockIdx.x / ngrp, slot0 = (blockIdx.x % ngrp) * SG, tid = threadIdx.x;
+    const int per = (nblk + 7) / 8;
+    const int L = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);   // consecutive on one XCD
+    if (L >= nblk) return;   // whole workgroup, before any barrier
+    const int t = L / ngrp, slot0 = (L % ngrp) * SG, tid = threadIdx.x;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(lds_raw);          // N
     cplx<T>* T1c = tw + N;                                       // [sl][kk][fj][fil]  SG*P*N*FC
     cplx<T>* T2c = T1c + SG * P * N * FC;                        // [sl][kk][jj][fil]  SG*P*P*FC
@@ -587,8 +592,9 @@ hipError_t down_p(int S, int ntar, const int32_t* tar_child, const double* tar_s
     static std::atomic<uint32_t> attr{0};
     hipError_t e = allow_large_lds((const void*)k_m2l_down<P, T>, attr);
     if (e != hipSuccess) return e;
-    k_m2l_down<P, T><<<ntar * (8 / SG), 256, lds, st>>>(S, SG, FC, tar_child, tar_scale, inv_lat, twiddle,
-                                                         static_cast<const cplx<T>*>(Pk), pck);
+    const int nblk = ntar * (8 / SG);
+    k_m2l_down<P, T><<<8 * ((nblk + 7) / 8), 256, lds, st>>>(S, SG, FC, nblk, tar_child, tar_scale, inv_lat, twiddle,
+                                                              static_cast<const cplx<T>*>(Pk), pck);
     return hipGetLastError();
 }
 

@@ -338,43 +338,143 @@ __global__ __launch_bounds__(64) void k_l2p_eval(int S, const double* __restrict
     }
 }
 
+// Float32 forms of the two kernels above for the mixed path's float32 operator applies: the same
+// waves and lanes, with coordinates taken relative to the leaf's centre (float32 differences of
+// values below 3 r), v_rsq_f32, float32 sums, float64 outputs.
+template <int KP>
+__global__ __launch_bounds__(64) void k_p2m_check_f32(int S, const double* __restrict__ usurf,
+                                                      const int32_t* __restrict__ leaf_cell, CellsDev cells,
+                                                      BodiesDev b, double* __restrict__ pc) {
+    __shared__ float4 sb[64];   // (x, y, z relative to the centre, q) of 64 bodies
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    float px[KP], py[KP], pz[KP], acc[KP];
+#pragma unroll
+    for (int m = 0; m < KP; m++) {
+        const int s = lane + 64 * m;
+        acc[m] = 0;
+        const int sr = s < S ? s : 0;   // slots past S evaluate point 0 and are not written
+        px[m] = (float)(usurf[3 * sr] * rr);
+        py[m] = (float)(usurf[3 * sr + 1] * rr);
+        pz[m] = (float)(usurf[3 * sr + 2] * rr);
+    }
+    for (int b0 = 0; b0 < bnum; b0 += 64) {
+        const int nb = min(64, bnum - b0);
+        __syncthreads();
+        if (lane < nb) {
+            const int bj = boff + b0 + lane;
+            sb[lane] = make_float4((float)(b.x[bj] - cx), (float)(b.y[bj] - cy), (float)(b.z[bj] - cz), (float)b.q[bj]);
+        }
+        __syncthreads();
+        for (int j = 0; j < nb; j++) {
+            const float4 e = sb[j];
+#pragma unroll
+            for (int m = 0; m < KP; m++) {
+                const float dx = px[m] - e.x, dy = py[m] - e.y, dz = pz[m] - e.z;
+                acc[m] += e.w * __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < KP; m++) {
+        const int s = lane + 64 * m;
+        if (s < S) pc[(size_t)li * S + s] = (double)acc[m];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_l2p_eval_f32(int S, const double* __restrict__ usurf,
+                                                     const int32_t* __restrict__ leaf_cell, CellsDev cells,
+                                                     BodiesDev b, const double* __restrict__ qdown,
+                                                     double* __restrict__ p, double* __restrict__ gx,
+                                                     double* __restrict__ gy, double* __restrict__ gz) {
+    extern __shared__ float4 eptf[];   // S equivalent points relative to the centre, and their densities
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int c = leaf_cell[li];
+    const double cx = cells.x[c], cy = cells.y[c], cz = cells.z[c], rr = cells.r[c] * 2.95;
+    for (int s = lane; s < S; s += 64)
+        eptf[s] = make_float4((float)(usurf[3 * s] * rr), (float)(usurf[3 * s + 1] * rr), (float)(usurf[3 * s + 2] * rr),
+                              (float)qdown[(size_t)li * S + s]);
+    __syncthreads();
+    const int boff = cells.boff[c], bnum = cells.bnum[c];
+    const int wb = slot_bits(bnum), W = 1 << wb, G = 64 >> wb;
+    const int g = lane >> wb, tl = lane & (W - 1);
+    for (int t0 = 0; t0 < bnum; t0 += W) {
+        const int i = t0 + tl;
+        const bool active = i < bnum;
+        const int bi = boff + (active ? i : 0);
+        const float xt = (float)(b.x[bi] - cx), yt = (float)(b.y[bi] - cy), zt = (float)(b.z[bi] - cz);
+        float ap = 0, ax = 0, ay = 0, az = 0;
+#pragma unroll 4
+        for (int s = g; s < S; s += G) {
+            const float4 e = eptf[s];
+            const float dx = xt - e.x, dy = yt - e.y, dz = zt - e.z;
+            const float ir = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+            float qi = e.w * ir;
+            ap += qi;
+            qi *= ir * ir;
+            ax += qi * dx;
+            ay += qi * dy;
+            az += qi * dz;
+        }
+        ap = fold_groups(ap, wb);
+        ax = fold_groups(ax, wb);
+        ay = fold_groups(ay, wb);
+        az = fold_groups(az, wb);
+        if (active && g == 0) {
+            p[bi] = (double)ap;
+            gx[bi] = -(double)ax;
+            gy[bi] = -(double)ay;
+            gz[bi] = -(double)az;
+        }
+    }
+}
+
 template <int KP>
 hipError_t p2m_kp(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells, BodiesDev b,
-                  double* pc, hipStream_t st) {
-    k_p2m_check<KP><<<nleaf, 64, 0, st>>>(S, usurf, leaf_cell, cells, b, pc);
+                  double* pc, bool f32, hipStream_t st) {
+    if (f32)
+        k_p2m_check_f32<KP><<<nleaf, 64, 0, st>>>(S, usurf, leaf_cell, cells, b, pc);
+    else
+        k_p2m_check<KP><<<nleaf, 64, 0, st>>>(S, usurf, leaf_cell, cells, b, pc);
     return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_p2m_check(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
-                            BodiesDev b, double* pc, hipStream_t st) {
+                            BodiesDev b, double* pc, hipStream_t st, bool f32) {
     if (nleaf <= 0) return hipSuccess;
     if (!usurf || !leaf_cell || !pc || !valid(cells) || !valid(b)) return hipErrorInvalidValue;
     switch ((S + 63) / 64) {
-        case 1: return p2m_kp<1>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 2: return p2m_kp<2>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 3: return p2m_kp<3>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 4: return p2m_kp<4>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 5: return p2m_kp<5>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 6: return p2m_kp<6>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 7: return p2m_kp<7>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 8: return p2m_kp<8>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 9: return p2m_kp<9>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 10: return p2m_kp<10>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 11: return p2m_kp<11>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
-        case 12: return p2m_kp<12>(S, usurf, leaf_cell, nleaf, cells, b, pc, st);
+        case 1: return p2m_kp<1>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 2: return p2m_kp<2>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 3: return p2m_kp<3>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 4: return p2m_kp<4>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 5: return p2m_kp<5>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 6: return p2m_kp<6>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 7: return p2m_kp<7>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 8: return p2m_kp<8>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 9: return p2m_kp<9>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 10: return p2m_kp<10>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 11: return p2m_kp<11>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
+        case 12: return p2m_kp<12>(S, usurf, leaf_cell, nleaf, cells, b, pc, f32, st);
         default: return hipErrorInvalidValue;   // S <= 728 (P <= 12)
     }
 }
 
 hipError_t launch_l2p_eval(int S, const double* usurf, const int32_t* leaf_cell, int nleaf, CellsDev cells,
                            BodiesDev b, const double* qdown, double* p, double* gx, double* gy, double* gz,
-                           hipStream_t st) {
+                           hipStream_t st, bool f32) {
     if (nleaf <= 0) return hipSuccess;
-    if (!usurf || !leaf_cell || !qdown || !p || !gx || !gy || !gz || !valid(cells) || !valid(b))
+    if (!usurf || !leaf_cell || !qdown || !p || !gx || !gy || !gz || !valid(cells) || !valid(b) || S <= 0 || S > 728)
         return hipErrorInvalidValue;
-    k_l2p_eval<<<nleaf, 64, 4 * S * sizeof(double), st>>>(S, usurf, leaf_cell, cells, b, qdown, p, gx, gy, gz);
+    if (f32)
+        k_l2p_eval_f32<<<nleaf, 64, (size_t)S * sizeof(float4), st>>>(S, usurf, leaf_cell, cells, b, qdown, p, gx, gy,
+                                                                       gz);
+    else
+        k_l2p_eval<<<nleaf, 64, 4 * S * sizeof(double), st>>>(S, usurf, leaf_cell, cells, b, qdown, p, gx, gy, gz);
     return hipGetLastError();
 }
 

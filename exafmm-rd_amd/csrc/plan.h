@@ -54,12 +54,18 @@ struct fmm3dpp_plan {
 
     DBuf d_inx, d_iny, d_inz, d_q, d_qin, d_p, d_gx, d_gy, d_gz, d_phi, d_grad;
     DBuf d_pn, d_gxn, d_gyn, d_gzn;   // near field, written by the P2P on the side stream
-    DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part;
+    DBuf d_pcleaf, d_pcleaf2, d_qeq, d_pck, d_Qk, d_Pk, d_partial, d_m2m_part, d_qcell;
     DBuf d_Qk_img, d_Pk_img, d_rel;   // mixed precision: float64 image-lattice spectra, float32 body records
     DBuf d_adj_phi, d_adj_grad, d_adj_w, d_adj_g, d_adj_dq, d_adj_dx;   // adjoint evaluation
     DeviceOperators dop;
 
     bool mixed() const { return cfg.precision == FMM3DPP_PREC_MIXED; }
+    // Mixed path: tree operator stages (P2M check, P2M / L2P pseudo-inverses, M2M, L2L, L2P
+    // evaluation) in float32, the applies on the float32 matrix cores, each cell's equivalent-density
+    // monopole set to its charge after P2M and M2M (d_qcell holds the charges), for the orders
+    // f32_apply_orders() admits (P = 6..8); off: float64 as in the float64 path (fmm3dpp_set_mixed_applies).
+    bool apply_f32 = true;
+    bool f32_applies() const { return mixed() && apply_f32 && f32_apply_orders(cfg.P); }
 
     CellsDev cells_dev() const {
         return {gs.cx.as<double>(), gs.cy.as<double>(), gs.cz.as<double>(), gs.cr.as<double>(), gs.depth.as<int32_t>(),

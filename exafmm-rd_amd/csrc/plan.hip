@@ -44,7 +44,7 @@ double now_ms() {
 }
 
 int upload_operators(fmm3dpp_plan* plan) {
-    HIPCK(plan->dop.upload(plan->ops, plan->mixed(), plan->stream));
+    HIPCK(plan->dop.upload(plan->ops, plan->mixed(), plan->apply_f32, plan->stream));
     HIPCK(hipStreamSynchronize(plan->stream));
     plan->ops_uploaded = true;
     return FMM3DPP_OK;
@@ -84,15 +84,16 @@ int stage_p2m_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t
     const int Sn = plan->ops.S;
     const int32_t* iota = plan->gs.iota.as<int32_t>();
     HIPCK(launch_p2m_check(Sn, plan->dop.o_usurf.as<double>(), leaf_cell, nleaf, plan->cells_dev(), plan->bodies_dev(),
-                           plan->d_pcleaf.as<double>(), s));
-    GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_ut.as<double>(), 0, plan->d_pcleaf.as<double>(), iota, iota, 0,
-                plan->d_pcleaf2.as<double>(), 0, nullptr};
-    g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-    HIPCK(launch_gemm_cols(g1, s));
-    GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vs.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, leaf_cell, 0,
-                plan->d_qeq.as<double>(), 0, leaf_depth};
-    g2.K = plan->dop.pinv_rank;
-    HIPCK(launch_gemm_cols(g2, s));
+                           plan->d_pcleaf.as<double>(), s, plan->f32_applies()));
+    const ApplyCols g1{Sn, nleaf, 1, false, plan->d_pcleaf.as<double>(), iota, iota, 0, plan->d_pcleaf2.as<double>(), 0,
+                       nullptr};
+    HIPCK(launch_apply(g1, plan->dop.lo_ut, s));
+    const ApplyCols g2{Sn, nleaf, 1, false, plan->d_pcleaf2.as<double>(), iota, leaf_cell, 0, plan->d_qeq.as<double>(), 0,
+                       leaf_depth};
+    HIPCK(launch_apply(g2, plan->dop.lo_vs, s));
+    if (plan->f32_applies())
+        HIPCK(launch_monopole_leaves(plan->d_qeq.as<double>(), Sn, leaf_cell, nleaf, plan->cells_dev(),
+                                     plan->d_q.as<double>(), plan->d_qcell.as<double>(), s));
     return FMM3DPP_OK;
 }
 
@@ -106,23 +107,26 @@ int total_charge(fmm3dpp_plan* plan, const double** qtot) {
     return FMM3DPP_OK;
 }
 
-// M2M (fmm.cpp:111-150, kernel.cpp:395-404, 434-442): per tree level from the deepest, one GEMM per
-// child octant into partials and the child sum; then the image levels.  In the infinite-lattice
+// M2M (fmm.cpp:111-150, kernel.cpp:395-404, 434-442): per tree level from the deepest, one apply per
+// child octant into partials and the child sum (and, with float32 applies, each parent's monopole set
+// to the sum of its children's charges); then the image levels.  In the infinite-lattice
 // gauge the root's and every image cell's monopole are set to their charge (the bodies' total times
 // 27^m for c(-m)), removing the rounding-level monopole the pseudo-inverse leaves.
 int stage_m2m_all(fmm3dpp_plan* plan) {
     hipStream_t s = plan->stream;
     const GpuStructure& g = plan->gs;
     const int Sn = plan->ops.S;
-    const size_t SS = (size_t)Sn * Sn;
     double* qeq = plan->d_qeq.as<double>();
     const int32_t* iota = g.iota.as<int32_t>();
     for (const auto& lv : g.m2m_levels) {
-        GemmCols gm{Sn, lv.ncol, 8, 0, plan->dop.o_m2m_all.as<double>(), (int64_t)SS, qeq,
-                    g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol, plan->d_m2m_part.as<double>(), 0, nullptr};
-        HIPCK(launch_gemm_cols(gm, s));
+        const ApplyCols gm{Sn, lv.ncol, 8, false, qeq, g.m2m_in.as<int32_t>() + lv.in_off, iota, lv.ncol,
+                           plan->d_m2m_part.as<double>(), 0, nullptr};
+        HIPCK(launch_apply(gm, plan->dop.lo_m2m, s));
         HIPCK(launch_sum_partials(Sn, lv.ncol, 8, plan->d_m2m_part.as<double>(), g.m2m_out.as<int32_t>() + lv.out_off,
                                   qeq, s));
+        if (plan->f32_applies())
+            HIPCK(launch_monopole_parents(qeq, Sn, g.m2m_out.as<int32_t>() + lv.out_off, lv.ncol,
+                                          g.m2m_in.as<int32_t>() + lv.in_off, plan->d_qcell.as<double>(), s));
     }
     const double* qtot = nullptr;   // the bodies' total charge (infinite-lattice gauge)
     if (plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL) {
@@ -131,9 +135,8 @@ int stage_m2m_all(fmm3dpp_plan* plan) {
         HIPCK(launch_neutralize(qeq, Sn, nullptr, 0, 1, qtot, 0, 1.0, s));
     }
     for (size_t i = 0; i < g.img_m2m.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, g.img_b.as<int32_t>() + i,
-                    g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
-        HIPCK(launch_gemm_cols(gi, s));
+        const ApplyCols gi{Sn, 1, 1, false, qeq, g.img_b.as<int32_t>() + i, g.img_a.as<int32_t>() + i, 0, qeq, 0, nullptr};
+        HIPCK(launch_apply(gi, plan->dop.lo_m2m_img, s));
         if (qtot) HIPCK(launch_neutralize(qeq, Sn, g.img_a.as<int32_t>() + i, 0, 1, qtot, 0, std::pow(27.0, i + 1), s));
     }
     return FMM3DPP_OK;
@@ -189,18 +192,17 @@ int stage_l2l_all(fmm3dpp_plan* plan) {
     hipStream_t s = plan->stream;
     const GpuStructure& g = plan->gs;
     const int Sn = plan->ops.S;
-    const size_t SS = (size_t)Sn * Sn;
     double* pck = plan->d_pck.as<double>();
     const size_t nimgm2m = g.img_m2m.size();
     for (size_t i = 0; i < g.img_l2l.size(); i++) {
-        GemmCols gi{Sn, 1, 1, 1, plan->dop.o_l2l_img.as<double>(), 0, pck, g.img_a.as<int32_t>() + nimgm2m + i,
-                    g.img_b.as<int32_t>() + nimgm2m + i, 0, pck, 0, nullptr};
-        HIPCK(launch_gemm_cols(gi, s));
+        const ApplyCols gi{Sn, 1, 1, true, pck, g.img_a.as<int32_t>() + nimgm2m + i, g.img_b.as<int32_t>() + nimgm2m + i,
+                           0, pck, 0, nullptr};
+        HIPCK(launch_apply(gi, plan->dop.lo_l2l_img, s));
     }
     for (const auto& lv : g.l2l_levels) {
-        GemmCols gl{Sn, lv.ncol, 8, 1, plan->dop.o_l2l.as<double>(), (int64_t)SS, pck, g.l2l_in.as<int32_t>() + lv.in_off,
-                    g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
-        HIPCK(launch_gemm_cols(gl, s));
+        const ApplyCols gl{Sn, lv.ncol, 8, true, pck, g.l2l_in.as<int32_t>() + lv.in_off,
+                           g.l2l_out.as<int32_t>() + lv.in_off, lv.ncol, pck, 0, nullptr};
+        HIPCK(launch_apply(gl, plan->dop.lo_l2l, s));
     }
     return FMM3DPP_OK;
 }
@@ -211,17 +213,15 @@ int stage_l2p_leaves(fmm3dpp_plan* plan, const int32_t* leaf_cell, const int32_t
     hipStream_t s = plan->stream;
     const int Sn = plan->ops.S;
     const int32_t* iota = plan->gs.iota.as<int32_t>();
-    GemmCols g1{Sn, nleaf, 1, 0, plan->dop.o_utl.as<double>(), 0, plan->d_pck.as<double>(), leaf_cell, iota, 0,
-                plan->d_pcleaf2.as<double>(), 0, nullptr};
-    g1.M = plan->dop.pinv_rank;   // truncated rank on the mixed path, full (0) in float64
-    HIPCK(launch_gemm_cols(g1, s));
-    GemmCols g2{Sn, nleaf, 1, 0, plan->dop.o_vsl.as<double>(), 0, plan->d_pcleaf2.as<double>(), iota, iota, 0,
-                plan->d_pcleaf.as<double>(), 0, leaf_depth};
-    g2.K = plan->dop.pinv_rank;
-    HIPCK(launch_gemm_cols(g2, s));
+    const ApplyCols g1{Sn, nleaf, 1, false, plan->d_pck.as<double>(), leaf_cell, iota, 0, plan->d_pcleaf2.as<double>(), 0,
+                       nullptr};
+    HIPCK(launch_apply(g1, plan->dop.lo_utl, s));
+    const ApplyCols g2{Sn, nleaf, 1, false, plan->d_pcleaf2.as<double>(), iota, iota, 0, plan->d_pcleaf.as<double>(), 0,
+                       leaf_depth};
+    HIPCK(launch_apply(g2, plan->dop.lo_vsl, s));
     HIPCK(launch_l2p_eval(Sn, plan->dop.o_usurf.as<double>(), leaf_cell, nleaf, plan->cells_dev(), plan->bodies_dev(),
                           plan->d_pcleaf.as<double>(), plan->d_p.as<double>(), plan->d_gx.as<double>(),
-                          plan->d_gy.as<double>(), plan->d_gz.as<double>(), s));
+                          plan->d_gy.as<double>(), plan->d_gz.as<double>(), s, plan->f32_applies()));
     return FMM3DPP_OK;
 }
 
@@ -254,6 +254,7 @@ int build_device(fmm3dpp_plan* plan, int64_t n, const double* dx, const double* 
     HIPCK(plan->d_qeq.ensure(nc * Sn * 8));
     HIPCK(hipMemsetAsync(plan->d_qeq.p, 0, nc * Sn * 8, s));
     HIPCK(plan->d_pck.ensure(nc * Sn * 8));
+    HIPCK(plan->d_qcell.ensure(nc * 8));
     if (plan->mixed()) {   // float32 spectra of the tree parents, float64 spectra of the image cells
         HIPCK(plan->d_Qk.ensure(F * std::max(g.nl, 1) * 8 * 8));
         HIPCK(plan->d_Pk.ensure(F * std::max(g.nl, 1) * 8 * 8));
@@ -401,6 +402,15 @@ int fmm3dpp_set_near_field_stream(fmm3dpp_plan* plan, int enable) {
 int fmm3dpp_set_hadamard_staging(fmm3dpp_plan* plan, int enable) {
     if (!plan) return FMM3DPP_ERR_ARG;
     plan->had_cap = enable ? kHadCap : -1;
+    return FMM3DPP_OK;
+}
+
+int fmm3dpp_set_mixed_applies(fmm3dpp_plan* plan, int float32) {
+    if (!plan) return FMM3DPP_ERR_ARG;
+    if (plan->apply_f32 != (float32 != 0)) {
+        plan->apply_f32 = float32 != 0;
+        plan->ops_uploaded = false;   // the operators' device layout depends on it
+    }
     return FMM3DPP_OK;
 }
 

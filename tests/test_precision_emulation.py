@@ -12,7 +12,10 @@ and with 3 pieces and 6 products, each product exact and summed in float32.  Fin
 * constraining each tree cell's equivalent-density monopole to the cell's charge (uniform shift of
   the density after P2M and M2M) removes the offset: float32 in all six stages stays below 1e-6;
 * BF16x3 under the same constraint is an order of magnitude worse than float32 (~1e-5 in the
-  potential, which is the whole error budget), so the matrix-core operator applies use float32.
+  potential, which is the whole error budget), so the matrix-core operator applies use float32;
+* the float32 stages hold from P = 6 to 8 only: at P = 4 the constraint moves the potential by the
+  float64 operators' own monopole truncation error (towards the direct sum), at P = 10 the float32
+  downward pass loses 2e-5 in the gradient of a periodic system.
 """
 import numpy as np
 import pytest
@@ -87,3 +90,61 @@ def test_bf16x3_is_much_worse_than_float32(emul):
           f"bf16x3 phi {b3[0]:.2e} grad {b3[2]:.2e}; bf16 3 pieces/6 products phi {b6[0]:.2e} grad {b6[2]:.2e}")
     assert b3[0] > 5 * f32[0] and b3[0] > 2e-6
     assert b6[0] < 2 * f32[0] + 2e-7
+
+
+def _case(oracle_mod, P, images, dipole, n=8192, seed=None):
+    rng = np.random.Generator(np.random.PCG64(2000 + P + images if seed is None else seed))
+    pts = rng.random((n, 3)) - 0.5
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    f = oracle_mod.OracleFMM(P, 64, images, 1.0, dipole=dipole, threads=8)
+    f.precompute()
+    xyz = [pts[:, k].copy() for k in range(3)]
+    f.set_particles(*xyz)
+    return f, xyz, q
+
+
+def _rank(f, P):
+    """The mixed path's truncated pseudo-inverse rank: singular values above 1e-7 of the largest,
+    rounded up to a multiple of 16."""
+    S = 6 * (P - 1) ** 2 + 2
+    n2 = (f.get_operator("VSinv").reshape(S, S) ** 2).sum(0)
+    s = np.where(n2 > 0, 1.0 / np.sqrt(np.where(n2 > 0, n2, 1.0)), 0.0)   # zero columns: dropped
+    r = int(np.nonzero(s > 1e-7 * s.max())[0].max() + 1)
+    return min(S, (r + 15) // 16 * 16)
+
+
+def test_float32_tree_stages_by_order(oracle_mod):
+    """The orders at which the mixed path runs its tree operator stages in float32 (P = 6..8):
+    * P = 6, 8: float32 in all six stages + the monopole constraint stays within 1e-6 of float64 (up
+      to a constant in the reference dipole mode with images, whose float64 root monopole the image
+      levels turn into an offset);
+    * P = 4: the monopole constraint moves the potential by the float64 operators' own monopole
+      truncation error (>5e-6, no images), towards the direct sum: no longer float64 in float32;
+    * P = 10: float32 L2L / L2P lose 1e-5 in the gradient of a periodic system."""
+    O = oracle_mod.OracleFMM
+    allx = (O.EMUL_P2M_CHECK | O.EMUL_P2M_PINV | O.EMUL_M2M | O.EMUL_L2L | O.EMUL_L2P_PINV | O.EMUL_L2P_EVAL
+            | O.EMUL_MONOPOLE)
+    down = O.EMUL_L2L | O.EMUL_L2P_PINV | O.EMUL_L2P_EVAL
+    for P, images in ((6, 2), (8, 3)):
+        f, _, q = _case(oracle_mod, P, images, 1)
+        p0, g0 = f.evaluate(q)
+        f.set_emulation(allx, O.EMUL_F32, _rank(f, P))
+        p, g = f.evaluate(q)
+        print(f"P={P} images={images}: phi up to a constant {_relc(p, p0):.2e} grad {_rel(g, g0):.2e}")
+        assert _relc(p, p0) < 1e-6 and _rel(g, g0) < 1e-6
+    f, (x, y, z), q = _case(oracle_mod, 4, 0, 0)
+    p0, g0 = f.evaluate(q)
+    f.set_emulation(allx, O.EMUL_F32, _rank(f, 4))
+    p, g = f.evaluate(q)
+    tar = np.arange(0, len(q), 4, dtype=np.int32)
+    pd, gd = oracle_mod.direct(x, y, z, q, 0, 1.0, targets=tar, threads=8)
+    print(f"P=4: float32 vs float64 phi {_rel(p, p0):.2e}; vs direct: float32 {_rel(p[tar], pd):.2e}, "
+          f"float64 {_rel(p0[tar], pd):.2e}")
+    assert _rel(p, p0) > 5e-6 and _rel(p[tar], pd) <= _rel(p0[tar], pd)
+    f, _, q = _case(oracle_mod, 10, 1, 1)
+    p0, g0 = f.evaluate(q)
+    f.set_emulation(down, O.EMUL_F32, _rank(f, 10))
+    p, g = f.evaluate(q)
+    print(f"P=10 images=1, float32 downward pass: grad {_rel(g, g0):.2e}")
+    assert _rel(g, g0) > 1e-5
