@@ -29,6 +29,7 @@ EXPORTS = (
     "fmm3dpp_get_operator", "fmm3dpp_set_operator", "fmm3dpp_get_stage_vectors", "fmm3dpp_timing_ms",
     "fmm3dpp_ewald", "fmm3dpp_ewald_device", "fmm3dpp_evaluate_adjoint", "fmm3dpp_evaluate_adjoint_device",
     "fmm3dpp_save_operators", "fmm3dpp_load_operators", "fmm3dpp_direct", "fmm3dpp_direct_device",
+    "fmm3dpp_direct_device_noalloc",
     "fmm3dpp_batch_create", "fmm3dpp_batch_destroy", "fmm3dpp_batch_last_error",
     "fmm3dpp_batch_set_particles_device", "fmm3dpp_batch_evaluate_device", "fmm3dpp_batch_set_graph",
     "fmm3dpp_batch_autotune", "fmm3dpp_batch_timing_ms",
@@ -101,6 +102,8 @@ def lib():
         L.fmm3dpp_direct.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_int32, _ip,
                                      ctypes.c_double, ctypes.c_int, _dp, _dp]
         L.fmm3dpp_direct_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int32, vp, ctypes.c_double,
+                                            ctypes.c_int, vp, vp]
+        L.fmm3dpp_direct_device_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int32, vp, ctypes.c_double,
                                             ctypes.c_int, vp, vp]
         L.fmm3dpp_save_operators.argtypes = [vp, ctypes.c_char_p]
         L.fmm3dpp_load_operators.argtypes = [vp, ctypes.c_char_p]

@@ -254,6 +254,16 @@ int fmm3dpp_direct_device(void* hip_stream, int64_t n, const double* d_x, const 
                           const double* d_q, int32_t ntar, const int32_t* d_targets, double cycle, int images,
                           double* d_phi, double* d_grad);
 
+/* No counterpart in the reference: fmm3dpp_direct_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results (the same kernels), but the per-chunk partial sums live in a static 16 MiB
+ * device array of the library instead of a fresh allocation: ntar <= 524,288 (FMM3DPP_ERR_ARG
+ * beyond), no hipMalloc / hipFree.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other host threads, or other streams before this call has returned) race. */
+int fmm3dpp_direct_device_noalloc(void* hip_stream, int64_t n, const double* d_x, const double* d_y,
+                                  const double* d_z, const double* d_q, int32_t ntar, const int32_t* d_targets,
+                                  double cycle, int images, double* d_phi, double* d_grad);
+
 /* ---- batches of independent periodic systems (no counterpart in the reference, whose LaplaceFMM
  *      solves one system; for callers with many systems of uneven size) ----
  * nsys configurations: images, cycle, precision, dipole_mode and divide_4pi must agree; P, ncrit and
