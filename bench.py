@@ -151,12 +151,26 @@ def run_clustered(pyfmm3dpp, torch, args):
     rmx = finish(rmx, True)
     r64 = finish(run_path(pyfmm3dpp, torch, sub, pyfmm3dpp.PREC_FP64, d_x, d_y, d_z, d_q, steps=1, warmup=1), False)
     leafn = ints[: c["n_tree_cells"], 6][ints[: c["n_tree_cells"], 4] == 0]
+    # the adaptive tree's accuracy: the float64 FMM over one image level (27 boxes, no dipole term)
+    # against the direct sum over the same boxes on 1024 sampled targets
+    tar = np.sort(np.random.Generator(np.random.PCG64(args.seed + 5)).choice(n, 1024, replace=False)).astype(np.int32)
+    p1 = pyfmm3dpp.LaplaceFMM(8, 128, 1, 1.0, dipole_mode=pyfmm3dpp.DIPOLE_OFF, device=0)
+    p1.precompute()
+    ph1, gr1 = p1.solve(x, y, z, q)
+    n_lost_images1 = int(p1.counts()["n_lost"])
+    p1.close()
+    pd, gd = pyfmm3dpp.direct(x, y, z, q, 1, 1.0, targets=tar)
+    vs_direct = {"phi": float("%.3e" % (np.linalg.norm(ph1[tar] - pd) / np.linalg.norm(pd))),
+                 "grad": float("%.3e" % (np.linalg.norm(gr1[tar] - gd) / np.linalg.norm(gd))),
+                 "targets": 1024, "images": 1, "n_lost": n_lost_images1}
     return {"workload": f"Plummer a=0.05 wrapped, N={n}, +-1 charges, P=8, ncrit=128, images=3, mixed",
             "particles_per_s": round(n / (rmx["ms_per_step"] / 1e3), 1), "ms_per_step": round(rmx["ms_per_step"], 3),
             "fp64_ms_per_step": round(r64["ms_per_step"], 3),
             "tree": {"cells": int(c["n_tree_cells"]), "leaves": int(len(leafn)),
                      "max_depth": int(ints[: c["n_tree_cells"], 2].max()),
-                     "leaf_bodies_min_mean_max": [int(leafn.min()), round(float(leafn.mean()), 1), int(leafn.max())]},
+                     "leaf_bodies_min_mean_max": [int(leafn.min()), round(float(leafn.mean()), 1), int(leafn.max())],
+                     "n_lost": int(c["n_lost"])},
+            "fp64_vs_direct": vs_direct,
             "breakdown_ms": rmx["stages"],
             "stage_rates": rmx["rates"],
             "phi_rel_l2_mixed_vs_fp64": float("%.3e" % rel_l2(torch, rmx["phi"], r64["phi"])),

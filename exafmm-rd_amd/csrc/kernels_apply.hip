@@ -201,6 +201,85 @@ __global__ __launch_bounds__(64) void k_apply_small(ApplyCols g, LaneOp op) {
     }
 }
 
+// Few columns, staged: the launch is one chain of dependent matrix ops per output block, so its time
+// is the chain plus the latency of its loads.  Four waves copy every operand of the block's chain --
+// the block's rows of all K chunks and the 16 columns' K values, both in step x lane order -- into
+// LDS with all loads in flight at once (about 40 per thread at P = 8), then wave 0 runs the chain
+// from LDS.  Same sequence of matrix ops per output element as k_apply and k_apply_small.
+template <class T>
+__global__ __launch_bounds__(256) void k_apply_small_lds(ApplyCols g, LaneOp op) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int nsteps = 4 * op.nkc;
+    T* As = reinterpret_cast<T*>(lds_raw);   // [step][lane]
+    T* Xs = As + (size_t)nsteps * 64;        // [step][lane]
+    int* src_s = reinterpret_cast<int*>(Xs + (size_t)nsteps * 64);   // the 16 columns' vectors
+    const int tid = threadIdx.x;
+    const int rb = blockIdx.x, vb = blockIdx.y, z = blockIdx.z;
+    const int M = op.M, K = op.K;
+    const T* A = static_cast<const T*>(op.p) + (int64_t)z * op.zstride + (int64_t)rb * 256;
+    const int64_t achunk = (int64_t)op.nmb * 256;
+    if (tid < 16) {
+        const int cv = vb * 16 + tid;
+        src_s[tid] = cv < g.ncol ? g.in_idx[(int64_t)z * g.idx_zstride + cv] : -1;
+    }
+    __syncthreads();
+    constexpr int UB = 16;   // loads issued before their stores: 2 x 16 per thread in flight
+    const int total = nsteps * 64;
+    for (int e0 = tid; e0 < total; e0 += 256 * UB) {
+        T ra[UB], rx[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int e = e0 + u * 256;
+            if (e < total) {
+                const int s = e >> 6, l = e & 63, k = 4 * s + (l >> 4), src = src_s[l & 15];
+                ra[u] = A[(int64_t)(s >> 2) * achunk + (s & 3) * 64 + l];
+                rx[u] = (src >= 0 && k < K) ? (T)g.X[(int64_t)src * g.S + k] : (T)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int e = e0 + u * 256;
+            if (e < total) {
+                As[e] = ra[u];
+                Xs[e] = rx[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid >= 64) return;   // wave 0 runs the chain (no barrier follows)
+    const int lane = tid, n = lane & 15, kq = lane >> 4;
+    acc_t acc = {0, 0, 0, 0};
+    for (int s0 = 0; s0 < nsteps; s0 += 4) {   // nsteps = 4 nkc: the next 4 steps' operands read first
+        T xa[4], aa[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            xa[u] = Xs[(s0 + u) * 64 + lane];
+            aa[u] = As[(s0 + u) * 64 + lane];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc = Mfma<T>::op(xa[u], aa[u], acc);
+    }
+    double* Y = g.Y + (int64_t)z * g.y_zstride;
+    const int32_t* out_idx = g.out_idx + (int64_t)z * g.idx_zstride;
+    const int r = rb * 16 + n;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = vb * 16 + Mfma<T>::row(kq, q);
+        if (c >= g.ncol || r >= M) continue;
+        const int oi = out_idx[c];
+        if (oi < 0) continue;
+        const double sc = g.scale_depth ? ldexp(1.0, -g.scale_depth[c]) : 1.0;
+        const double v = sc * (double)acc[q];
+        double* y = &Y[(int64_t)oi * g.S + r];
+        *y = g.accumulate ? *y + v : v;
+    }
+}
+
+template <class T>
+size_t small_lds(const LaneOp& op) { return (size_t)op.nkc * 4 * 64 * 2 * sizeof(T) + 16 * sizeof(int); }
+constexpr size_t kMaxLds = 160 * 1024;
+
 // Sum over a wave in a fixed order: lane-strided partial sums, then an xor butterfly after which
 // every lane holds the same value.
 __device__ __forceinline__ double wave_sum(double a) {
@@ -269,7 +348,11 @@ template <class T>
 hipError_t run_apply(const ApplyCols& g, const LaneOp& op, int mbw, hipStream_t st) {
     if (g.ncol <= kSmallCols && mbw <= 0) {
         dim3 grid(op.nmb, (g.ncol + 15) / 16, g.nz);
-        k_apply_small<T><<<grid, 64, 0, st>>>(g, op);
+        const size_t lds = small_lds<T>(op);
+        if (lds <= kMaxLds)
+            k_apply_small_lds<T><<<grid, 256, lds, st>>>(g, op);
+        else   // operands of more than 40 K chunks (float64 at P >= 11): straight from L2
+            k_apply_small<T><<<grid, 64, 0, st>>>(g, op);
         return hipGetLastError();
     }
     if (mbw <= 0) {
@@ -303,6 +386,11 @@ hipError_t launch_apply(const ApplyCols& g, const LaneOp& op, hipStream_t st, in
 
 hipError_t apply_init() {
     hipError_t e = hipSuccess;
+    for (const void* fn : {reinterpret_cast<const void*>(k_apply_small_lds<double>),
+                           reinterpret_cast<const void*>(k_apply_small_lds<float>)}) {
+        hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+        if (r != hipSuccess) e = r;
+    }
     for (hipError_t r : {tiled_init<double, 20, 4>(), tiled_init<double, 12, 4>(), tiled_init<double, 8, 4>(),
                          tiled_init<double, 4, 4>(), tiled_init<float, 20, 4>(), tiled_init<float, 12, 4>(),
                          tiled_init<float, 8, 4>(), tiled_init<float, 4, 4>()})
