@@ -272,9 +272,10 @@ def run_batch(pyfmm3dpp, torch, args):
 
 def stage_rooflines(plan, stages, mixed):
     """Achieved rates of each stage from its algorithmic work (counted from the plan's own tree and
-    lists) and its HIP-event time: flop rates against the matrix-core peak for the operator applies
-    (GEMM flops over the whole stage time, so P2M also pays its check potentials and L2P its field
-    evaluation) and the Hadamard, bytes for the transforms and the near field."""
+    lists) and its HIP-event time: flop rates against the matrix-core peak of the arithmetic the stage
+    runs in for the operator applies (GEMM flops over the whole stage time, so P2M also pays its check
+    potentials and L2P its field evaluation) and the Hadamard, bytes for the transforms and the near
+    field."""
     ints, dbl = plan.cells()
     c = plan.counts()
     ntree = c["n_tree_cells"]
@@ -286,13 +287,23 @@ def stage_rooflines(plan, stages, mixed):
     bn = ints[:, 6].astype(np.int64)
     pairs = float(np.sum(bn[tar] * bn[src]))
     flops_op = 2.0 * S * S
-    gemm = {"P2M": 2 * flops_op * nleaf, "L2P": 2 * flops_op * nleaf,
-            "M2M": flops_op * (ntree - 1), "L2L": flops_op * (ntree - 1)}
+    # the mixed path applies the P2M / L2P pseudo-inverses at the truncated rank (singular values
+    # above 1e-7 of the largest, rounded up to 16) and, for P = 6..8, all tree applies in float32
+    rank = S
+    f32_applies = mixed and 6 <= plan.P <= 8
+    if mixed:
+        vs = plan.get_operator("VSinv").reshape(S, S)
+        n2 = (vs * vs).sum(0)
+        sv = np.where(n2 > 0, 1.0 / np.sqrt(np.where(n2 > 0, n2, 1.0)), 0.0)
+        rank = min(S, (int(np.nonzero(sv > 1e-7 * sv.max())[0].max()) + 1 + 15) // 16 * 16)
+    pinv = 2 * (2.0 * rank * S)
+    gemm = {"P2M": pinv * nleaf, "L2P": pinv * nleaf, "M2M": flops_op * (ntree - 1), "L2L": flops_op * (ntree - 1)}
+    gpeak = FP32_PEAK_TFLOPS if f32_applies else FP64_PEAK_TFLOPS
     out = {}
     for k, fl in gemm.items():
         t = stages[k] * 1e-3
-        out[k] = {"gemm_gflop": round(fl / 1e9, 2), "tflops": round(fl / t / 1e12, 2),
-                  "frac_fp64_mfma": round(fl / t / 1e12 / FP64_PEAK_TFLOPS, 3)}
+        out[k] = {"gemm_gflop": round(fl / 1e9, 2), "tflops": round(fl / t / 1e12, 2), "peak_tflops": gpeak,
+                  "frac_mfma": round(fl / t / 1e12 / gpeak, 3)}
     had, npairs = hadamard_flops(plan)
     t = stages["M2L_hadamard"] * 1e-3
     peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
