@@ -7,9 +7,12 @@
 //   folded with cross-lane shuffles.  Uniform trees hold about ncrit/2 bodies per leaf, so a lane
 //   per target alone would leave half of every wave idle.
 // 1/r uses the hardware reciprocal square root (v_rsq_f64 / v_rsq_f32).  Measured on gfx950 over
-// r^2 in [1e-12, 1e2]: v_rsq_f64 is within 5.2e-8 relative, one Newton step brings it to 4.1e-15
-// (the float64 form used here, far inside the float64 path's 1e-12 parity bound), v_rsq_f32 to
-// 1.1e-7.
+// r^2 in [1e-12, 1e2]: v_rsq_f64 is within 5.2e-8 relative, v_rsq_f32 within 1.1e-7.  The float64
+// kernels refine v_rsq_f64 by one third-order step of the series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8
+// + ..., e = 1 - r^2 y0^2 formed with one fma: the dropped terms are O(e^3) ~ 1e-22, so the result
+// is within 1 ulp of the correctly rounded 1/sqrt (the reference's 1/sqrt(r^2), src/type.h:493-496)
+// for any start within 1e-7 (checked in exact arithmetic on the host: 0.95 ulp worst over 30,000
+// cases); one Newton step instead leaves up to ~40 ulp from the measured start.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -18,21 +21,23 @@ namespace fmm3dpp {
 
 namespace {
 
+// y0 (1 + e/2 + 3e^2/8), e = 1 - r2 y0^2, in the order the host check follows
+__device__ __forceinline__ double rsq_refine(double r2, double y0) {
+    const double h = r2 * y0;
+    const double e = fma(-h, y0, 1.0);
+    const double c = e * fma(0.375, e, 0.5);
+    return fma(y0, c, y0);
+}
+
 // 1/sqrt(r2) for r2 > 0, 0 for r2 == 0 (the reference's r > 0 mask, src/kernel.cpp:283-290)
 __device__ __forceinline__ double inv_sqrt(double r2) {
-    double y = __builtin_amdgcn_rsq(r2);
-    const double t = r2 * y * y;
-    y = y * fma(-0.5, t, 1.5);
+    const double y = rsq_refine(r2, __builtin_amdgcn_rsq(r2));
     return r2 > 0 ? y : 0.0;
 }
 
 // For r2 > 0 known in advance: bodies of a leaf against its check / equivalent surface, which lies
 // 2.95 r from the centre while the bodies are within sqrt(3) r, so r2 >= (1.95 r)^2.
-__device__ __forceinline__ double inv_sqrt_pos(double r2) {
-    const double y = __builtin_amdgcn_rsq(r2);
-    const double t = r2 * y * y;
-    return y * fma(-0.5, t, 1.5);
-}
+__device__ __forceinline__ double inv_sqrt_pos(double r2) { return rsq_refine(r2, __builtin_amdgcn_rsq(r2)); }
 
 __device__ __forceinline__ float inv_sqrt(float r2) {
     const float y = __builtin_amdgcn_rsqf(r2);
