@@ -352,6 +352,8 @@ struct Plan {
     // emul_kind the arithmetic (EmulKind), pinv_rank > 0 applies the P2M and L2P pseudo-inverses
     // at that rank (leading rows of UT / columns of VSinv), as the mixed path does.
     int emul_mask = 0, emul_kind = 0, pinv_rank = 0;
+    // emul_p2p (EmulP2P): the near field in float32 in one of three forms, see stage_p2p_emul.
+    int emul_p2p = 0;
     std::string err;
     // bodies in tree order
     int n = 0;
@@ -1036,8 +1038,82 @@ static void stage_l2p(Plan& pl, const std::vector<int>& leaves, double* p, doubl
     }
 }
 
+// Float32 near field (no counterpart in the reference; models the mixed path's P2P).  Forms:
+// EMUL_P2P_CELLREL: coordinates relative to the body's own cell centre rounded to float32, the
+//   source cell's displacement (its centre + periodic offset - the target cell's centre) formed in
+//   float64 and rounded once, charges rounded to float32, float32 arithmetic, and the pairs of
+//   one (target, piece of 64 source bodies, lane group j mod G) summed in float32 before a float64
+//   accumulator takes them (G = 64 / the target cell's slot count 8, 16, 32 or 64), as the GPU
+//   kernel does for leaf cells;
+// EMUL_P2P_CELLREL_F64SUM: the same values, every pair added straight into float64 (a perfectly
+//   compensated summation);
+// EMUL_P2P_DISP: the displacement x_t - x_s - offset formed in float64 and rounded to float32
+//   (no coordinate rounding beyond the displacement's own), float32 charges and arithmetic,
+//   float64 sums.
+enum EmulP2P { EMUL_P2P_OFF = 0, EMUL_P2P_CELLREL = 1, EMUL_P2P_CELLREL_F64SUM = 2, EMUL_P2P_DISP = 3 };
+
+static void stage_p2p_emul(Plan& pl, const double* q, double* p, double* gx, double* gy, double* gz) {
+    const int mode = pl.emul_p2p;
+    int maxd = 0;
+    for (auto& c : pl.cells) maxd = std::max(maxd, c.depth);
+    for (int d = 0; d <= maxd; d++) {   // non-leaf targets share bodies with their leaves: a depth at a time
+    std::vector<int> ts;
+    for (int t = 0; t < (int)pl.p2p.size(); t++)
+        if (!pl.p2p[t].empty() && pl.cells[t].depth == d) ts.push_back(t);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int ti = 0; ti < (int)ts.size(); ti++) {
+        const int t = ts[ti];
+        const auto& lst = pl.p2p[t];
+        const Cell& ct = pl.cells[t];
+        const int W = ct.bnum <= 8 ? 8 : ct.bnum <= 16 ? 16 : ct.bnum <= 32 ? 32 : 64, G = 64 / W;
+        for (int b = ct.boff; b < ct.boff + ct.bnum; b++) {
+            const float tx = (float)(pl.bx[b] - ct.x[0]), ty = (float)(pl.by[b] - ct.x[1]),
+                        tz = (float)(pl.bz[b] - ct.x[2]);
+            double ap = 0, ax = 0, ay = 0, az = 0;
+            for (auto& e : lst) {
+                const Cell& cs = pl.cells[e.src];
+                const float ox = (float)(cs.x[0] + e.off[0] - ct.x[0]), oy = (float)(cs.x[1] + e.off[1] - ct.x[1]),
+                            oz = (float)(cs.x[2] + e.off[2] - ct.x[2]);
+                const float xo = tx - ox, yo = ty - oy, zo = tz - oz;
+                for (int k0 = 0; k0 < cs.bnum; k0 += 64) {
+                    const int cnt = std::min(64, cs.bnum - k0);
+                    for (int g = 0; g < G; g++) {
+                        float fp = 0, fx = 0, fy = 0, fz = 0;
+                        for (int j = g; j < cnt; j += G) {
+                            const int s = cs.boff + k0 + j;
+                            float dx, dy, dz;
+                            if (mode == EMUL_P2P_DISP) {
+                                dx = (float)(pl.bx[b] - (pl.bx[s] + e.off[0]));
+                                dy = (float)(pl.by[b] - (pl.by[s] + e.off[1]));
+                                dz = (float)(pl.bz[b] - (pl.bz[s] + e.off[2]));
+                            } else {
+                                dx = xo - (float)(pl.bx[s] - cs.x[0]);
+                                dy = yo - (float)(pl.by[s] - cs.x[1]);
+                                dz = zo - (float)(pl.bz[s] - cs.x[2]);
+                            }
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            const float ir = r2 > 0 ? 1.0f / std::sqrt(r2) : 0.0f;
+                            float qi = (float)q[s] * ir;
+                            const float qi3 = qi * (ir * ir);
+                            if (mode == EMUL_P2P_CELLREL) {
+                                fp += qi; fx += qi3 * dx; fy += qi3 * dy; fz += qi3 * dz;
+                            } else {
+                                ap += qi; ax += (double)(qi3 * dx); ay += (double)(qi3 * dy); az += (double)(qi3 * dz);
+                            }
+                        }
+                        ap += fp; ax += fx; ay += fy; az += fz;
+                    }
+                }
+            }
+            p[b] += ap; gx[b] -= ax; gy[b] -= ay; gz[b] -= az;
+        }
+    }
+    }
+}
+
 // fmm.cpp:259-294 with kernel.cpp:258-311 (targets may be non-leaf: process one depth at a time)
 static void stage_p2p(Plan& pl, const double* q, double* p, double* gx, double* gy, double* gz) {
+    if (pl.emul_p2p) { stage_p2p_emul(pl, q, p, gx, gy, gz); return; }
     int maxd = 0;
     for (auto& c : pl.cells) maxd = std::max(maxd, c.depth);
     for (int d = 0; d <= maxd; d++) {
@@ -1116,6 +1192,7 @@ int orc_set_option(void* h, const char* name, int v) {
     if (!strcmp(name, "emulate")) { pl->emul_mask = v; return 0; }
     if (!strcmp(name, "emulate_kind")) { pl->emul_kind = v; return v >= EMUL_F32 && v <= EMUL_BF16X6 ? 0 : -1; }
     if (!strcmp(name, "pinv_rank")) { pl->pinv_rank = v; return 0; }
+    if (!strcmp(name, "emulate_p2p")) { pl->emul_p2p = v; return v >= EMUL_P2P_OFF && v <= EMUL_P2P_DISP ? 0 : -1; }
     return -1;
 }
 

@@ -108,6 +108,8 @@ class OracleFMM:
     # Stage bits and arithmetic of the reduced-precision emulation (orc_set_option "emulate").
     EMUL_P2M_CHECK, EMUL_P2M_PINV, EMUL_M2M, EMUL_L2L, EMUL_L2P_PINV, EMUL_L2P_EVAL, EMUL_MONOPOLE = 1, 2, 4, 8, 16, 32, 64
     EMUL_F32, EMUL_BF16X3, EMUL_BF16X6 = 0, 1, 2
+    # Forms of the float32 near field (orc_set_option "emulate_p2p"), see stage_p2p_emul.
+    EMUL_P2P_OFF, EMUL_P2P_CELLREL, EMUL_P2P_CELLREL_F64SUM, EMUL_P2P_DISP = 0, 1, 2, 3
 
     def set_emulation(self, mask, kind=0, pinv_rank=0):
         """Run the stages in `mask` in the emulated arithmetic `kind` (float32, or split bfloat16
@@ -115,6 +117,11 @@ class OracleFMM:
         for name, v in ((b"emulate", mask), (b"emulate_kind", kind), (b"pinv_rank", pinv_rank)):
             if lib().orc_set_option(self.h, name, int(v)) != 0:
                 raise ValueError(f"bad emulation option {name.decode()}={v}")
+
+    def set_p2p_emulation(self, form):
+        """Run the near field in float32 in `form` (EMUL_P2P_*; 0 = float64)."""
+        if lib().orc_set_option(self.h, b"emulate_p2p", int(form)) != 0:
+            raise ValueError(f"bad near-field emulation form {form}")
 
     def set_particles(self, x, y, z):
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))

@@ -15,7 +15,10 @@ and with 3 pieces and 6 products, each product exact and summed in float32.  Fin
   potential, which is the whole error budget), so the matrix-core operator applies use float32;
 * the float32 stages hold from P = 6 to 8 only: at P = 4 the constraint moves the potential by the
   float64 operators' own monopole truncation error (towards the direct sum), at P = 10 the float32
-  downward pass loses 2e-5 in the gradient of a periodic system.
+  downward pass loses 2e-5 in the gradient of a periodic system;
+* the float32 near field's gradient error (4e-6) is the rounding of cell-relative coordinates to
+  float32, not its float32 sums: a perfectly compensated sum leaves it unchanged, a displacement
+  formed in float64 and rounded once brings it to 1e-7.
 """
 import numpy as np
 import pytest
@@ -148,3 +151,25 @@ def test_float32_tree_stages_by_order(oracle_mod):
     p, g = f.evaluate(q)
     print(f"P=10 images=1, float32 downward pass: grad {_rel(g, g0):.2e}")
     assert _rel(g, g0) > 1e-5
+
+
+def test_float32_near_field_error_is_the_coordinate_rounding(oracle_mod):
+    """The mixed path's gradient error comes from its float32 P2P, and there from rounding the
+    cell-relative coordinates to float32 (an absolute error of ~3e-8 of the cell size, a large
+    relative error in the displacement of the closest pairs, which 1/r^2 weighs most), not from
+    float32 summation: summing every pair in float64 (a perfectly compensated sum) leaves the error
+    where it is, while rounding only the float64-formed displacement removes it."""
+    O = oracle_mod.OracleFMM
+    f, _, q = _case(oracle_mod, 6, 1, 1, n=32768, seed=7)
+    p0, g0 = f.evaluate(q)
+    err = {}
+    for form in (O.EMUL_P2P_CELLREL, O.EMUL_P2P_CELLREL_F64SUM, O.EMUL_P2P_DISP):
+        f.set_p2p_emulation(form)
+        p, g = f.evaluate(q)
+        err[form] = (_rel(p, p0), _rel(g, g0))
+    f.set_p2p_emulation(O.EMUL_P2P_OFF)
+    print("float32 near field, phi / grad vs float64: cell-relative coordinates {:.2e} / {:.2e}; the same with "
+          "float64 sums {:.2e} / {:.2e}; float64 displacement rounded once {:.2e} / {:.2e}".format(
+              *err[O.EMUL_P2P_CELLREL], *err[O.EMUL_P2P_CELLREL_F64SUM], *err[O.EMUL_P2P_DISP]))
+    cell, summed, disp = err[O.EMUL_P2P_CELLREL][1], err[O.EMUL_P2P_CELLREL_F64SUM][1], err[O.EMUL_P2P_DISP][1]
+    assert cell > 1e-6 and abs(summed - cell) < 0.1 * cell and disp < 0.1 * cell
