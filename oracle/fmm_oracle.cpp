@@ -130,8 +130,12 @@ static inline void gemv(const double* A, const double* x, double* y, int m, int 
 // ---- reduced-precision emulation (test infrastructure for the mixed-precision path) ----
 // EMUL_MONOPOLE: every tree cell's equivalent density is shifted uniformly so that its sum (the
 // monopole) equals the charge of the cell's bodies, after P2M (leaves) and M2M (parents).
+// EMUL_M2L: the M2L of tree parents (depth >= 0) in float32: equivalent densities and twiddles rounded
+// to float32, up transform, 8x8 Hadamard (the spectra, the kernel tables and every partial sum in
+// float32, the target spectra accumulated in float32 over the patterns) and down transform in float32,
+// the check potentials accumulated in float64; the image cells' M2L stays float64.
 enum EmulBits { EMUL_P2M_CHECK = 1, EMUL_P2M_PINV = 2, EMUL_M2M = 4, EMUL_L2L = 8, EMUL_L2P_PINV = 16,
-                EMUL_L2P_EVAL = 32, EMUL_MONOPOLE = 64 };
+                EMUL_L2P_EVAL = 32, EMUL_MONOPOLE = 64, EMUL_M2L = 128 };
 
 // Uniform shift of v[0..S) that makes its sum equal `charge`.
 static void set_monopole(double* v, int S, double charge) {
@@ -267,57 +271,72 @@ static void svd_jacobi(const std::vector<double>& A, int n, std::vector<double>&
 struct Dft {
     int N, Nh;
     std::vector<cplx> w;  // w[m] = exp(-2 pi i m / N)
-    explicit Dft(int n = 1) : N(n), Nh(n / 2 + 1), w(n) {
+    std::vector<std::complex<float>> wf;  // w rounded to float32 (reduced-precision emulation)
+    explicit Dft(int n = 1) : N(n), Nh(n / 2 + 1), w(n), wf(n) {
         for (int m = 0; m < n; m++) w[m] = cplx(std::cos(2 * M_PI * m / n), -std::sin(2 * M_PI * m / n));
+        for (int m = 0; m < n; m++) wf[m] = std::complex<float>((float)w[m].real(), (float)w[m].imag());
     }
     // full r2c of a real N^3 grid g[k][j][i] -> G[k][j][i'], i' < Nh  (FFTW forward, unnormalised)
-    void r2c(const double* g, cplx* G, int nz_lim = -1) const {
+    void r2c(const double* g, cplx* G, int nz_lim = -1) const { r2c_t<double>(g, G, nz_lim, w); }
+    // the same in float32 arithmetic (inputs and twiddles rounded to float32)
+    void r2c_f32(const double* g, cplx* G, int nz_lim = -1) const { r2c_t<float>(g, G, nz_lim, wf); }
+    // c2r (FFTW backward, unnormalised) evaluated only at i,j,k in [lo, N)
+    void c2r(const cplx* G, double* g, int lo = 0) const { c2r_t<double>(G, g, lo, w); }
+    void c2r_f32(const cplx* G, double* g, int lo = 0) const { c2r_t<float>(G, g, lo, wf); }
+
+    template <class T>
+    void r2c_t(const double* g, cplx* G, int nz_lim, const std::vector<std::complex<T>>& w) const {
+        typedef std::complex<T> ct;
         int lim = nz_lim < 0 ? N : nz_lim;  // inputs nonzero only for i,j,k < lim
-        std::vector<cplx> t1((size_t)N * N * Nh, cplx(0, 0)), t2((size_t)N * N * Nh, cplx(0, 0));
+        std::vector<ct> t1((size_t)N * N * Nh, ct(0, 0)), t2((size_t)N * N * Nh, ct(0, 0));
         for (int k = 0; k < lim; k++)
             for (int j = 0; j < lim; j++)
                 for (int f = 0; f < Nh; f++) {
-                    cplx a(0, 0);
-                    for (int i = 0; i < lim; i++) a += g[((size_t)k * N + j) * N + i] * w[(i * f) % N];
+                    ct a(0, 0);
+                    for (int i = 0; i < lim; i++) a += (T)g[((size_t)k * N + j) * N + i] * w[(i * f) % N];
                     t1[((size_t)k * N + j) * Nh + f] = a;
                 }
         for (int k = 0; k < lim; k++)
             for (int fj = 0; fj < N; fj++)
                 for (int f = 0; f < Nh; f++) {
-                    cplx a(0, 0);
+                    ct a(0, 0);
                     for (int j = 0; j < lim; j++) a += t1[((size_t)k * N + j) * Nh + f] * w[(j * fj) % N];
                     t2[((size_t)k * N + fj) * Nh + f] = a;
                 }
         for (int fk = 0; fk < N; fk++)
             for (int fj = 0; fj < N; fj++)
                 for (int f = 0; f < Nh; f++) {
-                    cplx a(0, 0);
+                    ct a(0, 0);
                     for (int k = 0; k < lim; k++) a += t2[((size_t)k * N + fj) * Nh + f] * w[(k * fk) % N];
-                    G[((size_t)fk * N + fj) * Nh + f] = a;
+                    G[((size_t)fk * N + fj) * Nh + f] = cplx(a.real(), a.imag());
                 }
     }
-    // c2r (FFTW backward, unnormalised) evaluated only at i,j,k in [lo, N)
-    void c2r(const cplx* G, double* g, int lo = 0) const {
-        std::vector<cplx> t1((size_t)N * N * Nh), t2((size_t)N * N * Nh);
+    template <class T>
+    void c2r_t(const cplx* G, double* g, int lo, const std::vector<std::complex<T>>& w) const {
+        typedef std::complex<T> ct;
+        std::vector<ct> t1((size_t)N * N * Nh), t2((size_t)N * N * Nh);
         for (int k = lo; k < N; k++)
             for (int fj = 0; fj < N; fj++)
                 for (int f = 0; f < Nh; f++) {
-                    cplx a(0, 0);
-                    for (int fk = 0; fk < N; fk++) a += G[((size_t)fk * N + fj) * Nh + f] * std::conj(w[(k * fk) % N]);
+                    ct a(0, 0);
+                    for (int fk = 0; fk < N; fk++) {
+                        const cplx& v = G[((size_t)fk * N + fj) * Nh + f];
+                        a += ct((T)v.real(), (T)v.imag()) * std::conj(w[(k * fk) % N]);
+                    }
                     t1[((size_t)k * N + fj) * Nh + f] = a;
                 }
         for (int k = lo; k < N; k++)
             for (int j = lo; j < N; j++)
                 for (int f = 0; f < Nh; f++) {
-                    cplx a(0, 0);
+                    ct a(0, 0);
                     for (int fj = 0; fj < N; fj++) a += t1[((size_t)k * N + fj) * Nh + f] * std::conj(w[(j * fj) % N]);
                     t2[((size_t)k * N + j) * Nh + f] = a;
                 }
         for (int k = lo; k < N; k++)
             for (int j = lo; j < N; j++)
                 for (int i = lo; i < N; i++) {
-                    const cplx* row = &t2[((size_t)k * N + j) * Nh];
-                    double a = row[0].real();
+                    const ct* row = &t2[((size_t)k * N + j) * Nh];
+                    T a = row[0].real();
                     for (int f = 1; f < Nh; f++) {
                         if (2 * f == N) { a += (row[f] * std::conj(w[(i * f) % N])).real(); continue; }
                         a += 2 * (row[f] * std::conj(w[(i * f) % N])).real();
@@ -836,7 +855,9 @@ static void stage_m2m(Plan& pl, int mind, int maxd, const double* qtot = nullptr
 }
 
 // kernel.cpp:1140-1397 (m2l_fft_precompute_t) with hadamard_8x8 (1012-1082) and hadamard_1x27 (1084-1138)
-static void stage_m2l(Plan& pl) {
+// F32: the tree parents' M2L in float32 (EMUL_M2L); the float64 form is the restatement.
+template <bool F32>
+static void stage_m2l_t(Plan& pl) {
     int S = pl.S, N = pl.N, N3 = pl.N3, Nh = pl.Nh, F = pl.F, P = pl.P;
     double ts0 = now();
     std::vector<int> tars;
@@ -894,7 +915,12 @@ static void stage_m2l(Plan& pl) {
             int slot = cc.octant == 13 ? 0 : cc.octant;
             std::fill(grid.begin(), grid.end(), 0.0);
             for (int s = 0; s < S; s++) grid[pl.up_idx[s]] = pl.qeq[(size_t)cc.idx * S + s];
-            dft.r2c(grid.data(), out.data(), P);
+            if constexpr (F32) {
+                if (cp.depth >= 0) dft.r2c_f32(grid.data(), out.data(), P);
+                else dft.r2c(grid.data(), out.data(), P);
+            } else {
+                dft.r2c(grid.data(), out.data(), P);
+            }
             for (int f = 0; f < F; f++) dst[(size_t)f * 8 + slot] = out[f];
         }
     }
@@ -909,6 +935,21 @@ static void stage_m2l(Plan& pl) {
                 if (sl < 0 || slot_img[t][pi] != 0) continue;
                 const cplx* qk = &Qk[sl * fsz + (size_t)f * 8];
                 cplx* pk = &Pk[t * fsz + (size_t)f * 8];
+                if constexpr (F32) {
+                    if (pl.cells[tars[t]].depth >= 0) {   // float32 kernel, spectra and sums
+                        for (int it = 0; it < 8; it++) {
+                            float ar = 0, ai = 0;
+                            for (int is = 0; is < 8; is++) {
+                                float gr = (float)g8[(is * 8 + it) * 2], gi = (float)g8[(is * 8 + it) * 2 + 1];
+                                float qr = (float)qk[is].real(), qi = (float)qk[is].imag();
+                                ar += gr * qr - gi * qi;
+                                ai += gr * qi + gi * qr;
+                            }
+                            pk[it] = cplx((float)pk[it].real() + ar, (float)pk[it].imag() + ai);
+                        }
+                        continue;
+                    }
+                }
                 for (int it = 0; it < 8; it++) {
                     double ar = 0, ai = 0;
                     for (int is = 0; is < 8; is++) {
@@ -953,7 +994,12 @@ static void stage_m2l(Plan& pl) {
             const Cell& cc = pl.cells[ct.coff + ch];
             int slot = cc.octant == 13 ? 0 : cc.octant;
             for (int f = 0; f < F; f++) in[f] = Pk[ti * fsz + (size_t)f * 8 + slot];
-            dft.c2r(in.data(), g.data(), P - 1);
+            if constexpr (F32) {
+                if (ct.depth >= 0) dft.c2r_f32(in.data(), g.data(), P - 1);
+                else dft.c2r(in.data(), g.data(), P - 1);
+            } else {
+                dft.c2r(in.data(), g.data(), P - 1);
+            }
             double* pc = &pl.pck[(size_t)cc.idx * S];
             for (int s = 0; s < S; s++) pc[s] += g[pl.down_idx[s]] / N3 * scale;
         }
@@ -962,6 +1008,11 @@ static void stage_m2l(Plan& pl) {
     pl.timers["m2l_up"] = t1 - t0;
     pl.timers["m2l_hadamard"] = t2 - t1;
     pl.timers["m2l_down"] = t3 - t2;
+}
+
+static void stage_m2l(Plan& pl) {
+    if (pl.emul_mask & EMUL_M2L) stage_m2l_t<true>(pl);
+    else stage_m2l_t<false>(pl);
 }
 
 // fmm.cpp:195-229 with kernel.cpp:1419-1435

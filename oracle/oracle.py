@@ -107,6 +107,7 @@ class OracleFMM:
 
     # Stage bits and arithmetic of the reduced-precision emulation (orc_set_option "emulate").
     EMUL_P2M_CHECK, EMUL_P2M_PINV, EMUL_M2M, EMUL_L2L, EMUL_L2P_PINV, EMUL_L2P_EVAL, EMUL_MONOPOLE = 1, 2, 4, 8, 16, 32, 64
+    EMUL_M2L = 128   # float32 M2L of the tree parents (up transform, 8x8 Hadamard, down transform)
     EMUL_F32, EMUL_BF16X3, EMUL_BF16X6 = 0, 1, 2
     # Forms of the float32 near field (orc_set_option "emulate_p2p"), see stage_p2p_emul.
     EMUL_P2P_OFF, EMUL_P2P_CELLREL, EMUL_P2P_CELLREL_F64SUM, EMUL_P2P_DISP = 0, 1, 2, 3

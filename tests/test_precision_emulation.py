@@ -18,7 +18,9 @@ and with 3 pieces and 6 products, each product exact and summed in float32.  Fin
   downward pass loses 2e-5 in the gradient of a periodic system;
 * the float32 near field's gradient error (4e-6) is the rounding of cell-relative coordinates to
   float32, not its float32 sums: a perfectly compensated sum leaves it unchanged, a displacement
-  formed in float64 and rounded once brings it to 1e-7.
+  formed in float64 and rounded once brings it to 1e-7;
+* with the float32 M2L emulated as well (EMUL_M2L), the whole mixed path stays within its bounds,
+  its gradient error is the P2P's and its potential error the far field's.
 """
 import numpy as np
 import pytest
@@ -173,3 +175,30 @@ def test_float32_near_field_error_is_the_coordinate_rounding(oracle_mod):
               *err[O.EMUL_P2P_CELLREL], *err[O.EMUL_P2P_CELLREL_F64SUM], *err[O.EMUL_P2P_DISP]))
     cell, summed, disp = err[O.EMUL_P2P_CELLREL][1], err[O.EMUL_P2P_CELLREL_F64SUM][1], err[O.EMUL_P2P_DISP][1]
     assert cell > 1e-6 and abs(summed - cell) < 0.1 * cell and disp < 0.1 * cell
+
+
+def test_whole_mixed_path_emulated(oracle_mod):
+    """Every float32 stage of the mixed path at once (M2L of the tree parents, the six tree stages with
+    the monopole constraint at the truncated rank, the cell-relative float32 P2P) against float64, in
+    the infinite-lattice gauge, P=8, 3 image levels: within the mixed path's bounds (2e-6 potential,
+    1e-5 gradient), the gradient set by the P2P and the potential by the M2L and the tree stages.
+    At the bench's N=2^20 inputs the same emulation gives 9.1e-7 / 3.96e-6 against 1.7e-6 / 3.93e-6
+    measured on the GPU (whose float32 transforms, matrix-op sums and v_rsq_f32 round differently)."""
+    O = oracle_mod.OracleFMM
+    f, _, q = _case(oracle_mod, 8, 3, 2, n=32768, seed=11)
+    p0, g0 = f.evaluate(q)
+    tree = O.EMUL_P2M_CHECK | O.EMUL_P2M_PINV | O.EMUL_M2M | O.EMUL_L2L | O.EMUL_L2P_PINV | O.EMUL_L2P_EVAL | O.EMUL_MONOPOLE
+    err = {}
+    for name, mask, p2p in (("m2l", O.EMUL_M2L, 0), ("tree", tree, 0), ("p2p", 0, O.EMUL_P2P_CELLREL),
+                            ("all", tree | O.EMUL_M2L, O.EMUL_P2P_CELLREL)):
+        f.set_emulation(mask, O.EMUL_F32, 192 if mask & tree else 0)
+        f.set_p2p_emulation(p2p)
+        p, g = f.evaluate(q)
+        err[name] = (_rel(p, p0), _rel(g, g0))
+    f.set_emulation(0, 0, 0)
+    f.set_p2p_emulation(O.EMUL_P2P_OFF)
+    print("float32 stages vs float64 (phi / grad): " + ", ".join(f"{k} {v[0]:.2e} / {v[1]:.2e}" for k, v in err.items()))
+    assert err["all"][0] < 2e-6 and err["all"][1] < 1e-5
+    assert err["p2p"][1] > 0.8 * err["all"][1]                      # the gradient is the P2P's
+    assert max(err["m2l"][0], err["tree"][0]) > 2 * err["p2p"][0]   # the potential is the far field's
+    assert err["m2l"][1] < 0.3 * err["all"][1] and err["tree"][1] < 0.3 * err["all"][1]
