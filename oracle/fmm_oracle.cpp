@@ -133,7 +133,8 @@ static inline void gemv(const double* A, const double* x, double* y, int m, int 
 // EMUL_M2L: the M2L of tree parents (depth >= 0) in float32: equivalent densities and twiddles rounded
 // to float32, up transform, 8x8 Hadamard (the spectra, the kernel tables and every partial sum in
 // float32, the target spectra accumulated in float32 over the patterns) and down transform in float32,
-// the check potentials accumulated in float64; the image cells' M2L stays float64.
+// the check potentials accumulated in float64; the image cells' M2L stays float64.  With a split
+// bfloat16 kind the Hadamard's products are formed as that scheme forms them (transforms float32).
 enum EmulBits { EMUL_P2M_CHECK = 1, EMUL_P2M_PINV = 2, EMUL_M2M = 4, EMUL_L2L = 8, EMUL_L2P_PINV = 16,
                 EMUL_L2P_EVAL = 32, EMUL_MONOPOLE = 64, EMUL_M2L = 128 };
 
@@ -165,6 +166,19 @@ static inline void bf16_split(double v, int pieces, float* out) {
         out[i] = bf16_round((float)r);
         r -= out[i];
     }
+}
+
+// a * b as the matrix cores' split-bfloat16 scheme `kind` forms it: both split into 2 (BF16X3) or 3
+// (BF16X6) bfloat16 pieces, the products of order < pieces, each exact, summed in float32.
+static inline float split_product(double a, double b, int kind) {
+    const int pieces = kind == EMUL_BF16X3 ? 2 : 3;
+    float as[3], bs[3];
+    bf16_split(a, pieces, as);
+    bf16_split(b, pieces, bs);
+    float s = 0;
+    for (int u = 0; u < pieces; u++)
+        for (int v = 0; u + v < pieces; v++) s += as[u] * bs[v];
+    return s;
 }
 
 // y[i] = k * sum_{j<n} A[i*lda + j] x[j] in the arithmetic `kind` (EmulKind), rounded back to double.
@@ -940,10 +954,15 @@ static void stage_m2l_t(Plan& pl) {
                         for (int it = 0; it < 8; it++) {
                             float ar = 0, ai = 0;
                             for (int is = 0; is < 8; is++) {
-                                float gr = (float)g8[(is * 8 + it) * 2], gi = (float)g8[(is * 8 + it) * 2 + 1];
-                                float qr = (float)qk[is].real(), qi = (float)qk[is].imag();
-                                ar += gr * qr - gi * qi;
-                                ai += gr * qi + gi * qr;
+                                const double gr = g8[(is * 8 + it) * 2], gi = g8[(is * 8 + it) * 2 + 1];
+                                const float qr = (float)qk[is].real(), qi = (float)qk[is].imag();
+                                if (pl.emul_kind == EMUL_F32) {
+                                    ar += (float)gr * qr - (float)gi * qi;
+                                    ai += (float)gr * qi + (float)gi * qr;
+                                } else {   // split bfloat16 products (matrix-core schemes), float32 sums
+                                    ar += split_product(gr, qr, pl.emul_kind) - split_product(gi, qi, pl.emul_kind);
+                                    ai += split_product(gr, qi, pl.emul_kind) + split_product(gi, qr, pl.emul_kind);
+                                }
                             }
                             pk[it] = cplx((float)pk[it].real() + ar, (float)pk[it].imag() + ai);
                         }

@@ -202,3 +202,23 @@ def test_whole_mixed_path_emulated(oracle_mod):
     assert err["p2p"][1] > 0.8 * err["all"][1]                      # the gradient is the P2P's
     assert max(err["m2l"][0], err["tree"][0]) > 2 * err["p2p"][0]   # the potential is the far field's
     assert err["m2l"][1] < 0.3 * err["all"][1] and err["tree"][1] < 0.3 * err["all"][1]
+
+
+def test_hadamard_split_bfloat16(oracle_mod):
+    """Which matrix-core arithmetic the 8x8 M2L Hadamard could use instead of float32: two bfloat16
+    pieces with 3 products ("BF16x3") put the potential 2e-5 off (measured 1.9e-5 here, 6e-5 at
+    N=2^20), beyond the whole 1e-5 budget; three pieces with 6 products are as accurate as float32
+    (the products are exact, only the float32 sums round)."""
+    O = oracle_mod.OracleFMM
+    f, _, q = _case(oracle_mod, 8, 3, 2, n=32768, seed=42)
+    p0, g0 = f.evaluate(q)
+    err = {}
+    for kind in (O.EMUL_F32, O.EMUL_BF16X3, O.EMUL_BF16X6):
+        f.set_emulation(O.EMUL_M2L, kind, 0)
+        p, g = f.evaluate(q)
+        err[kind] = (_rel(p, p0), _rel(g, g0))
+    f.set_emulation(0, 0, 0)
+    print("M2L Hadamard, phi / grad vs float64: float32 {:.2e} / {:.2e}, bf16x3 {:.2e} / {:.2e}, bf16 3 pieces / "
+          "6 products {:.2e} / {:.2e}".format(*err[O.EMUL_F32], *err[O.EMUL_BF16X3], *err[O.EMUL_BF16X6]))
+    assert err[O.EMUL_BF16X3][0] > 1e-5 and err[O.EMUL_BF16X3][0] > 10 * err[O.EMUL_F32][0]
+    assert err[O.EMUL_BF16X6][0] < 1.5 * err[O.EMUL_F32][0] and err[O.EMUL_BF16X6][1] < 1.5 * err[O.EMUL_F32][1]
