@@ -10,6 +10,9 @@
 //                (3dpp/src/kernel.cpp:1290-1295,1347-1352,1725)
 //   * GEMV     : plain loops — replaces cblas_dgemv (3dpp/src/mathfunc.cpp:11-23)
 // Pinning of each part is documented in the tests that exercise it (tests/test_oracle_*.py).
+// It also emulates the mixed-precision path's reduced-precision arithmetic stage by stage (float32,
+// split bfloat16; the "emulate*" options, no counterpart in the reference), so that the error each
+// stage would add can be measured on the host (tests/test_precision_emulation.py).
 #include <algorithm>
 #include <array>
 #include <cmath>
