@@ -1,8 +1,8 @@
 // TEST INFRASTRUCTURE ONLY — never linked into, loaded by, or called from the product path.
 //
 // CPU restatement (C++17 + OpenMP, FP64) of the `3dpp` periodic kernel-independent FMM
-// evaluation path of mumairsadiq/exafmm-rd.  It is the parity checker for the HIP product
-// (tests/, __graft_entry__.smoke()) and the reported CPU baseline (bench.py cpu_baseline leg).
+// evaluation path of mumairsadiq/exafmm-rd.  It is the oracle the HIP product's parity tests compare
+// against (tests/, __graft_entry__.smoke()) and the reported CPU baseline (bench.py cpu_baseline leg).
 // Every function names the reference file:line it follows.  The reference's third-party
 // numerics (LAPACK dgesvd, FFTW r2c/c2r, CBLAS dgemv) are absent here and are restated:
 //   * SVD      : one-sided Jacobi (Hestenes) — replaces dgesvd_ (3dpp/src/mathfunc.cpp:81-110)
@@ -1395,7 +1395,7 @@ int orc_evaluate(void* h, const double* qin, double* pout, double* gxo, double* 
     return 0;
 }
 
-// One stage of orc_evaluate at a time (the checker of the library's per-stage entry points):
+// One stage of orc_evaluate at a time (what the library's per-stage entry points are compared with):
 // "charges" (qin, input order; clears the per-cell vectors and body fields), "p2m", "m2m", "m2l",
 // "l2l", "l2p" (body fields = far field), "p2p" (body fields += near field), "p2p_only" (body
 // fields = near field).  Each stage reads the state the previous calls left.

@@ -92,7 +92,7 @@ def test_fmm_approximates_finite_lattice(oracle_mod, P, images, tol_f):
 
 @pytest.mark.parametrize("dipole", [1, 2])
 def test_oracle_stages_equal_evaluate(oracle_mod, dipole):
-    """The oracle's per-stage form (the checker of the library's per-stage entry points) runs the
+    """The oracle's per-stage form (what the library's per-stage entry points are compared with) runs the
     same code as its evaluate: the stage sequence reproduces evaluate bit for bit."""
     rng = np.random.Generator(np.random.PCG64(3))
     n = 3000
