@@ -101,3 +101,35 @@ def test_gpu_structure_reused_plan(oracle_mod):
         depths.append(int(pi[: plan.counts()["n_tree_cells"], 2].max()))
     print("tree depths", depths, flush=True)
     assert depths[1] > depths[0] + 2   # the clustered tree is deeper than the levels sorted for it
+
+
+def _sorted_rows(cols):
+    a = np.stack([np.asarray(c, dtype=np.float64) for c in cols], axis=1)
+    return a[np.lexsort(a.T[::-1])]
+
+
+def test_host_structure_deep_plummer_tree(oracle_mod):
+    """262,144 bodies of a Plummer sphere (scale radius 0.05, wrapped into the unit cube), ncrit 32,
+    3 image levels: an adaptive tree nine or more levels deep, host build vs the restated reference,
+    bit for bit, with no lost leaf pairs."""
+    import sys
+    import os
+    import pyfmm3dpp
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    x, y, z, _ = bench.plummer_workload(1 << 18, 19)
+    orc = oracle_mod.OracleFMM(4, 32, 3, 1.0)
+    orc.set_particles(x, y, z)
+    plan = pyfmm3dpp.LaplaceFMM(4, 32, 3, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    plan.set_particles(x, y, z)
+    oi, od = orc.cells()
+    pi, pd = plan.cells()
+    ntree = plan.counts()["n_tree_cells"]
+    assert int(pi[:ntree, 2].max()) >= 9
+    assert np.array_equal(oi, pi) and np.array_equal(od, pd)
+    assert np.array_equal(orc.body_order(), plan.body_order())
+    for lo, lp in ((orc.p2p_list(), plan.p2p_list()), (orc.m2l_list(), plan.m2l_list())):
+        ro = _sorted_rows([lo[0], lo[1]] + [lo[2][:, k] for k in range(3)] + list(lo[3:]))
+        rp = _sorted_rows([lp[0], lp[1]] + [lp[2][:, k] for k in range(3)] + list(lp[3:]))
+        assert np.array_equal(ro, rp)
+    assert plan.counts()["n_lost"] == orc.lost_pairs() == 0
