@@ -514,7 +514,7 @@ int ensure_operators(fmm3dpp_batch* b) {
         Group& g = *gp;
         if (g.ops.P == 0) {
             double t0 = now_ms();
-            int rc = build_operators(g.P, b->cfg[0].images, b->cfg[0].cycle, g.ops, b->err);
+            int rc = build_operators_cached(g.P, b->cfg[0].images, b->cfg[0].cycle, g.ops, b->err);
             if (rc) return rc;
             b->ms["precompute"] += now_ms() - t0;
         }

@@ -7,7 +7,10 @@
 // element count, elements; finally a u64 FNV-1a hash of every byte before it.  Every field of
 // Operators is stored, the derived device forms included, so a loaded set is bit-identical to the
 // one that was saved.
+#include <unistd.h>
+
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <type_traits>
@@ -21,6 +24,8 @@ namespace fmm3dpp {
 namespace {
 
 constexpr char kMagic[12] = "FMM3DPP-OPS";
+// Raised whenever build_operators' results change, so that files of an older computation (in an
+// FMM3DPP_OPERATOR_CACHE directory, say) are refused rather than used.
 constexpr uint32_t kVersion = 1;
 
 struct Hash {
@@ -186,6 +191,26 @@ int load_operators(Operators& ops, int P, int images, double cycle, const std::s
     for (int k = 0; k < 27; k++) sizes = sizes && o.m2m_img[k].size() == SSi;
     if (!sizes) { err = path + ": operator sizes do not match P=" + std::to_string(P); return FMM3DPP_ERR_ARG; }
     ops = std::move(o);
+    return FMM3DPP_OK;
+}
+
+int build_operators_cached(int P, int images, double cycle, Operators& ops, std::string& err) {
+    const char* dir = std::getenv("FMM3DPP_OPERATOR_CACHE");
+    if (!dir || !*dir) return build_operators(P, images, cycle, ops, err);
+    uint64_t cbits;
+    std::memcpy(&cbits, &cycle, 8);
+    char name[128];
+    snprintf(name, sizeof(name), "/fmm3dpp_ops_v%u_P%d_i%d_c%016llx.bin", (unsigned)kVersion, P, images,
+             (unsigned long long)cbits);
+    const std::string path = std::string(dir) + name;
+    std::string ignored;
+    if (load_operators(ops, P, images, cycle, path, ignored) == FMM3DPP_OK) return FMM3DPP_OK;
+    const int rc = build_operators(P, images, cycle, ops, err);
+    if (rc != FMM3DPP_OK) return rc;
+    // written under a name of this process, then renamed: a reader sees no file or a whole one
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    if (save_operators(ops, cycle, tmp, ignored) != FMM3DPP_OK || std::rename(tmp.c_str(), path.c_str()) != 0)
+        std::remove(tmp.c_str());
     return FMM3DPP_OK;
 }
 

@@ -57,7 +57,7 @@ namespace fmm3dpp {
 int ensure_ops(fmm3dpp_plan* plan) {
     if (!plan->ops_ready) {
         double t0 = now_ms();
-        int rc = build_operators(plan->cfg.P, plan->cfg.images, plan->cfg.cycle, plan->ops, plan->err);
+        int rc = build_operators_cached(plan->cfg.P, plan->cfg.images, plan->cfg.cycle, plan->ops, plan->err);
         if (rc) return rc;
         plan->host_ms["precompute"] = now_ms() - t0;
         plan->ops_ready = true;

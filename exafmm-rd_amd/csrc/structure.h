@@ -60,6 +60,11 @@ void derive_fused(Operators& ops);
 // Operator cache file (every field, bit-exact); load checks the configuration and a checksum.
 int save_operators(const Operators& ops, double cycle, const std::string& path, std::string& err);
 int load_operators(Operators& ops, int P, int images, double cycle, const std::string& path, std::string& err);
+// build_operators through the directory named by the environment variable FMM3DPP_OPERATOR_CACHE
+// when it is set: the operators of (P, images, cycle) are loaded from their file there when it is
+// present and valid (configuration and checksum), otherwise computed and written there (a directory
+// that cannot be written costs only the file).  Without the variable, build_operators.
+int build_operators_cached(int P, int images, double cycle, Operators& ops, std::string& err);
 
 inline int surface_count(int P) { return 6 * (P - 1) * (P - 1) + 2; }
 

@@ -74,7 +74,10 @@ int fmm3dpp_plan_create(const fmm3dpp_config* cfg, int device, fmm3dpp_plan** ou
 int fmm3dpp_plan_destroy(fmm3dpp_plan* plan);
 const char* fmm3dpp_last_error(const fmm3dpp_plan* plan);
 
-/* One-time operator precompute for (P, images, cycle); implied by the first evaluate. */
+/* One-time operator precompute for (P, images, cycle); implied by the first evaluate.  When the
+ * environment variable FMM3DPP_OPERATOR_CACHE names a directory, the operators are loaded from the
+ * file of this (P, images, cycle) there if it is present and valid, and otherwise computed and
+ * written there, so that processes after the first skip the computation (results bit-identical). */
 int fmm3dpp_precompute(fmm3dpp_plan* plan);
 
 /* Builds the adaptive octree and the interaction lists for n bodies (host arrays, input order).

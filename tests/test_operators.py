@@ -175,3 +175,39 @@ def test_operator_cache_round_trip(tmp_path):
         c.load_operators(bad)
     with pytest.raises(pyfmm3dpp.FMMError):
         c.load_operators(str(tmp_path / "missing.bin"))
+
+
+def test_operator_cache_directory(tmp_path, monkeypatch):
+    """FMM3DPP_OPERATOR_CACHE: the first precompute of a (P, images, cycle) writes its file to the
+    directory, a later plan loads it instead of computing (bit-identical operators); a damaged file
+    is recomputed and replaced; a directory that cannot be written only loses the file."""
+    import glob
+    import pyfmm3dpp
+    names = pyfmm3dpp.OPERATOR_NAMES + pyfmm3dpp.IMAGE_OPERATOR_NAMES + ["GK27", "GK27_SUM", "M2M_IMG_SUM"]
+    ref = pyfmm3dpp.LaplaceFMM(6, 16, 2, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+    ref.precompute()                                   # no cache directory: computed
+    monkeypatch.setenv("FMM3DPP_OPERATOR_CACHE", str(tmp_path))
+
+    def plan():
+        p = pyfmm3dpp.LaplaceFMM(6, 16, 2, 1.0, device=pyfmm3dpp.DEVICE_NONE)
+        p.precompute()
+        for name in names:
+            assert np.array_equal(p.get_operator(name), ref.get_operator(name)), name
+        return p.timing_ms("precompute")
+
+    t_first = plan()
+    files = glob.glob(str(tmp_path / "fmm3dpp_ops_*.bin"))
+    assert len(files) == 1 and not glob.glob(str(tmp_path / "*.tmp*"))
+    t_loaded = plan()
+    print(f"precompute {t_first:.0f} ms, from the cache directory {t_loaded:.0f} ms")
+    assert t_loaded < 0.5 * t_first
+    data = bytearray(open(files[0], "rb").read())
+    data[len(data) // 3] ^= 0x10
+    open(files[0], "wb").write(bytes(data))
+    plan()                                             # damaged: recomputed, file replaced
+    plan()                                             # and loaded again
+    other = pyfmm3dpp.LaplaceFMM(6, 16, 2, 2.0, device=pyfmm3dpp.DEVICE_NONE)
+    other.precompute()                                 # another cycle: a file of its own
+    assert len(glob.glob(str(tmp_path / "fmm3dpp_ops_*.bin"))) == 2
+    monkeypatch.setenv("FMM3DPP_OPERATOR_CACHE", str(tmp_path / "missing" / "dir"))
+    plan()
