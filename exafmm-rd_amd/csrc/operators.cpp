@@ -26,7 +26,10 @@ namespace {
 
 using ld = long double;
 
-// Minimal persistent pool: parallel_for(n, f) runs f(i) for i in [0, n) on all workers.
+// Minimal persistent pool: parallel_for(n, f) runs f(i) for i in [0, n) on all workers.  The
+// precompute issues thousands of short jobs in a row (one per round of Jacobi rotations), so a worker
+// that finished a job spins briefly for the next one before sleeping, and the caller spins briefly
+// for the workers to finish before waiting on the condition variable.
 class Pool {
 public:
     Pool() {
@@ -38,7 +41,7 @@ public:
         {
             std::lock_guard<std::mutex> g(m_);
             stop_ = true;
-            gen_++;
+            gen_.fetch_add(1, std::memory_order_release);
         }
         cv_.notify_all();
         for (auto& w : workers_) w.join();
@@ -48,15 +51,23 @@ public:
         if (nthreads_ == 1 || n == 1) { for (int64_t i = 0; i < n; i++) f(i); return; }
         {
             std::lock_guard<std::mutex> g(m_);
-            job_ = &f; n_ = n; next_ = 0; busy_ = (int)nthreads_ - 1; gen_++;
+            job_ = &f;
+            n_ = n;
+            next_.store(0, std::memory_order_relaxed);
+            busy_.store((int)nthreads_ - 1, std::memory_order_relaxed);
+            gen_.fetch_add(1, std::memory_order_release);
         }
         cv_.notify_all();
         run();
-        std::unique_lock<std::mutex> lk(m_);
-        done_.wait(lk, [this] { return busy_ == 0; });
+        for (int k = 0; k < kSpin && busy_.load(std::memory_order_acquire) != 0; k++) std::this_thread::yield();
+        if (busy_.load(std::memory_order_acquire) != 0) {
+            std::unique_lock<std::mutex> lk(m_);
+            done_.wait(lk, [this] { return busy_.load(std::memory_order_acquire) == 0; });
+        }
         job_ = nullptr;
     }
 private:
+    static constexpr int kSpin = 2000;
     void run() {
         for (;;) {
             int64_t i = next_.fetch_add(1);
@@ -67,16 +78,17 @@ private:
     void loop() {
         uint64_t seen = 0;
         for (;;) {
-            {
+            for (int k = 0; k < kSpin && gen_.load(std::memory_order_acquire) == seen; k++) std::this_thread::yield();
+            if (gen_.load(std::memory_order_acquire) == seen) {
                 std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return gen_ != seen; });
-                seen = gen_;
-                if (stop_) return;
+                cv_.wait(lk, [&] { return gen_.load(std::memory_order_acquire) != seen; });
             }
+            seen = gen_.load(std::memory_order_acquire);
+            if (stop_) return;
             run();
-            {
+            if (busy_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
                 std::lock_guard<std::mutex> g(m_);
-                if (--busy_ == 0) done_.notify_one();
+                done_.notify_one();
             }
         }
     }
@@ -87,8 +99,8 @@ private:
     const std::function<void(int64_t)>* job_ = nullptr;
     int64_t n_ = 0;
     std::atomic<int64_t> next_{0};
-    int busy_ = 0;
-    uint64_t gen_ = 0;
+    std::atomic<int> busy_{0};
+    std::atomic<uint64_t> gen_{0};
     bool stop_ = false;
 };
 
@@ -122,15 +134,33 @@ void laplace_matrix(const std::vector<double>& src, const std::vector<double>& t
         }
 }
 
-// C = A (m x k) * B (k x n), long double, rows in parallel.
+// C = A (m x k) * B (k x n), long double, rows in parallel.  Every element is summed over l in
+// order from 0; four elements of a row are held in registers while l runs.
 void matmul(const std::vector<ld>& A, const std::vector<ld>& B, int m, int k, int n, std::vector<ld>& C) {
     C.assign((size_t)m * n, 0.0L);
     pool().parallel_for(m, [&](int64_t i) {
         ld* c = &C[(size_t)i * n];
-        for (int l = 0; l < k; l++) {
-            ld a = A[(size_t)i * k + l];
-            const ld* b = &B[(size_t)l * n];
-            for (int j = 0; j < n; j++) c[j] += a * b[j];
+        const ld* a = &A[(size_t)i * k];
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+            ld c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+            for (int l = 0; l < k; l++) {
+                const ld al = a[l];
+                const ld* b = &B[(size_t)l * n + j];
+                c0 += al * b[0];
+                c1 += al * b[1];
+                c2 += al * b[2];
+                c3 += al * b[3];
+            }
+            c[j] = c0;
+            c[j + 1] = c1;
+            c[j + 2] = c2;
+            c[j + 3] = c3;
+        }
+        for (; j < n; j++) {
+            ld cj = 0;
+            for (int l = 0; l < k; l++) cj += a[l] * B[(size_t)l * n + j];
+            c[j] = cj;
         }
     });
 }
