@@ -131,7 +131,7 @@ uint64_t expected_size(const std::string& name, int P, int images) {
 int save_operators(const Operators& ops, double cycle, const std::string& path, std::string& err) {
     FILE* f = fopen(path.c_str(), "wb");
     if (!f) { err = "cannot open " + path + " for writing"; return FMM3DPP_ERR_ARG; }
-    Writer w{f};
+    Writer w{f, Hash{}};
     w.put(kMagic, sizeof(kMagic));
     w.val<uint32_t>(kVersion);
     w.val<int32_t>(ops.P);
@@ -148,7 +148,7 @@ int save_operators(const Operators& ops, double cycle, const std::string& path, 
 int load_operators(Operators& ops, int P, int images, double cycle, const std::string& path, std::string& err) {
     FILE* f = fopen(path.c_str(), "rb");
     if (!f) { err = "cannot open " + path; return FMM3DPP_ERR_ARG; }
-    Reader r{f};
+    Reader r{f, Hash{}};
     char magic[sizeof(kMagic)];
     r.get(magic, sizeof(magic));
     const uint32_t version = r.val<uint32_t>();

@@ -383,7 +383,7 @@ int build_operators(int P, int images, double cycle, Operators& op, std::string&
     op.l2l_img = images > 0 ? transpose_d(op.m2m_img[13], S, S) : std::vector<double>();
 
     // M2L kernels on the convolution grid (kernel.cpp:1706-1850)
-    const int N = op.N, Nh = op.Nh, F = op.F;
+    const int N = op.N, F = op.F;
     const double delta = 2 * r0 / (P - 1) * 1.05, gsize = r0 * 2 * 1.05;
     std::vector<ld> wr(N), wi(N);
     op.twiddle.resize(2 * N);

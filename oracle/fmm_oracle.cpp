@@ -302,7 +302,7 @@ struct Dft {
     void c2r_f32(const cplx* G, double* g, int lo = 0) const { c2r_t<float>(G, g, lo, wf); }
 
     template <class T>
-    void r2c_t(const double* g, cplx* G, int nz_lim, const std::vector<std::complex<T>>& w) const {
+    void r2c_t(const double* g, cplx* G, int nz_lim, const std::vector<std::complex<T>>& tw) const {
         typedef std::complex<T> ct;
         int lim = nz_lim < 0 ? N : nz_lim;  // inputs nonzero only for i,j,k < lim
         std::vector<ct> t1((size_t)N * N * Nh, ct(0, 0)), t2((size_t)N * N * Nh, ct(0, 0));
@@ -310,26 +310,26 @@ struct Dft {
             for (int j = 0; j < lim; j++)
                 for (int f = 0; f < Nh; f++) {
                     ct a(0, 0);
-                    for (int i = 0; i < lim; i++) a += (T)g[((size_t)k * N + j) * N + i] * w[(i * f) % N];
+                    for (int i = 0; i < lim; i++) a += (T)g[((size_t)k * N + j) * N + i] * tw[(i * f) % N];
                     t1[((size_t)k * N + j) * Nh + f] = a;
                 }
         for (int k = 0; k < lim; k++)
             for (int fj = 0; fj < N; fj++)
                 for (int f = 0; f < Nh; f++) {
                     ct a(0, 0);
-                    for (int j = 0; j < lim; j++) a += t1[((size_t)k * N + j) * Nh + f] * w[(j * fj) % N];
+                    for (int j = 0; j < lim; j++) a += t1[((size_t)k * N + j) * Nh + f] * tw[(j * fj) % N];
                     t2[((size_t)k * N + fj) * Nh + f] = a;
                 }
         for (int fk = 0; fk < N; fk++)
             for (int fj = 0; fj < N; fj++)
                 for (int f = 0; f < Nh; f++) {
                     ct a(0, 0);
-                    for (int k = 0; k < lim; k++) a += t2[((size_t)k * N + fj) * Nh + f] * w[(k * fk) % N];
+                    for (int k = 0; k < lim; k++) a += t2[((size_t)k * N + fj) * Nh + f] * tw[(k * fk) % N];
                     G[((size_t)fk * N + fj) * Nh + f] = cplx(a.real(), a.imag());
                 }
     }
     template <class T>
-    void c2r_t(const cplx* G, double* g, int lo, const std::vector<std::complex<T>>& w) const {
+    void c2r_t(const cplx* G, double* g, int lo, const std::vector<std::complex<T>>& tw) const {
         typedef std::complex<T> ct;
         std::vector<ct> t1((size_t)N * N * Nh), t2((size_t)N * N * Nh);
         for (int k = lo; k < N; k++)
@@ -338,7 +338,7 @@ struct Dft {
                     ct a(0, 0);
                     for (int fk = 0; fk < N; fk++) {
                         const cplx& v = G[((size_t)fk * N + fj) * Nh + f];
-                        a += ct((T)v.real(), (T)v.imag()) * std::conj(w[(k * fk) % N]);
+                        a += ct((T)v.real(), (T)v.imag()) * std::conj(tw[(k * fk) % N]);
                     }
                     t1[((size_t)k * N + fj) * Nh + f] = a;
                 }
@@ -346,7 +346,7 @@ struct Dft {
             for (int j = lo; j < N; j++)
                 for (int f = 0; f < Nh; f++) {
                     ct a(0, 0);
-                    for (int fj = 0; fj < N; fj++) a += t1[((size_t)k * N + fj) * Nh + f] * std::conj(w[(j * fj) % N]);
+                    for (int fj = 0; fj < N; fj++) a += t1[((size_t)k * N + fj) * Nh + f] * std::conj(tw[(j * fj) % N]);
                     t2[((size_t)k * N + j) * Nh + f] = a;
                 }
         for (int k = lo; k < N; k++)
@@ -355,8 +355,8 @@ struct Dft {
                     const ct* row = &t2[((size_t)k * N + j) * Nh];
                     T a = row[0].real();
                     for (int f = 1; f < Nh; f++) {
-                        if (2 * f == N) { a += (row[f] * std::conj(w[(i * f) % N])).real(); continue; }
-                        a += 2 * (row[f] * std::conj(w[(i * f) % N])).real();
+                        if (2 * f == N) { a += (row[f] * std::conj(tw[(i * f) % N])).real(); continue; }
+                        a += 2 * (row[f] * std::conj(tw[(i * f) % N])).real();
                     }
                     g[((size_t)k * N + j) * N + i] = a;
                 }
@@ -704,7 +704,7 @@ static void precompute(Plan& pl) {
         for (int j = 0; j < S; j++) pl.VSinv_l2p[(size_t)i * S + j] = U[(size_t)i * S + j] * sinv[j];
 
     // precompute_m2l: Gk of 1/r on the conv grid for every |rel|_max in [2, range]
-    int N = pl.N, Nh = pl.Nh, F = pl.F;
+    int N = pl.N, F = pl.F;
     double delta = 2 * r0 / (P - 1) * 1.05, gsize = r0 * 2 * 1.05;   // surface.h:58-86
     int range = pl.images >= 2 ? 4 : 3;
     auto rels = relx_list(range, 2);
@@ -875,7 +875,7 @@ static void stage_m2m(Plan& pl, int mind, int maxd, const double* qtot = nullptr
 // F32: the tree parents' M2L in float32 (EMUL_M2L); the float64 form is the restatement.
 template <bool F32>
 static void stage_m2l_t(Plan& pl) {
-    int S = pl.S, N = pl.N, N3 = pl.N3, Nh = pl.Nh, F = pl.F, P = pl.P;
+    int S = pl.S, N = pl.N, N3 = pl.N3, F = pl.F, P = pl.P;
     double ts0 = now();
     std::vector<int> tars;
     std::set<int> srcset;
