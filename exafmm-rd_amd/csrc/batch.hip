@@ -725,7 +725,6 @@ int fmm3dpp_batch_autotune(fmm3dpp_batch* b, const int32_t* candidates, int32_t 
     const bool graph = b->use_graph;
     b->use_graph = false;
     b->drop_graph();
-    hipStream_t s = b->stream;
     for (auto& gp : b->groups) {
         Group& g = *gp;
         double best = 1e300;
