@@ -1122,8 +1122,21 @@ static void stage_l2p(Plan& pl, const std::vector<int>& leaves, double* p, doubl
 //   compensated summation);
 // EMUL_P2P_DISP: the displacement x_t - x_s - offset formed in float64 and rounded to float32
 //   (no coordinate rounding beyond the displacement's own), float32 charges and arithmetic,
-//   float64 sums.
-enum EmulP2P { EMUL_P2P_OFF = 0, EMUL_P2P_CELLREL = 1, EMUL_P2P_CELLREL_F64SUM = 2, EMUL_P2P_DISP = 3 };
+//   float64 sums;
+// EMUL_P2P_TWOFLOAT: every cell-relative coordinate and the cells' offset held as two float32
+//   values (hi + lo); per (target, list entry) the target's position relative to the source cell is
+//   formed with an exact float32 two-sum of the hi parts, per pair dx = (xo_hi - s_hi) + (xo_lo -
+//   s_lo) in float32 (the hi difference of a close pair is exact), float32 arithmetic after it,
+//   sums as in EMUL_P2P_CELLREL.
+enum EmulP2P { EMUL_P2P_OFF = 0, EMUL_P2P_CELLREL = 1, EMUL_P2P_CELLREL_F64SUM = 2, EMUL_P2P_DISP = 3,
+               EMUL_P2P_TWOFLOAT = 4 };
+
+// a + b = s + e exactly, s = fl(a + b) (Knuth's two-sum in float32)
+static inline void two_sum_f(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
 
 static void stage_p2p_emul(Plan& pl, const double* q, double* p, double* gx, double* gy, double* gz) {
     const int mode = pl.emul_p2p;
@@ -1142,12 +1155,27 @@ static void stage_p2p_emul(Plan& pl, const double* q, double* p, double* gx, dou
         for (int b = ct.boff; b < ct.boff + ct.bnum; b++) {
             const float tx = (float)(pl.bx[b] - ct.x[0]), ty = (float)(pl.by[b] - ct.x[1]),
                         tz = (float)(pl.bz[b] - ct.x[2]);
+            const float tlx = (float)(pl.bx[b] - ct.x[0] - tx), tly = (float)(pl.by[b] - ct.x[1] - ty),
+                        tlz = (float)(pl.bz[b] - ct.x[2] - tz);
             double ap = 0, ax = 0, ay = 0, az = 0;
             for (auto& e : lst) {
                 const Cell& cs = pl.cells[e.src];
                 const float ox = (float)(cs.x[0] + e.off[0] - ct.x[0]), oy = (float)(cs.x[1] + e.off[1] - ct.x[1]),
                             oz = (float)(cs.x[2] + e.off[2] - ct.x[2]);
                 const float xo = tx - ox, yo = ty - oy, zo = tz - oz;
+                float xh = 0, yh = 0, zh = 0, xl = 0, yl = 0, zl = 0;   // EMUL_P2P_TWOFLOAT
+                if (mode == EMUL_P2P_TWOFLOAT) {
+                    const double odx = cs.x[0] + e.off[0] - ct.x[0], ody = cs.x[1] + e.off[1] - ct.x[1],
+                                 odz = cs.x[2] + e.off[2] - ct.x[2];
+                    const float olx = (float)(odx - ox), oly = (float)(ody - oy), olz = (float)(odz - oz);
+                    float ex, ey, ez;
+                    two_sum_f(tx, -ox, xh, ex);
+                    two_sum_f(ty, -oy, yh, ey);
+                    two_sum_f(tz, -oz, zh, ez);
+                    xl = (tlx - olx) + ex;
+                    yl = (tly - oly) + ey;
+                    zl = (tlz - olz) + ez;
+                }
                 for (int k0 = 0; k0 < cs.bnum; k0 += 64) {
                     const int cnt = std::min(64, cs.bnum - k0);
                     for (int g = 0; g < G; g++) {
@@ -1159,6 +1187,13 @@ static void stage_p2p_emul(Plan& pl, const double* q, double* p, double* gx, dou
                                 dx = (float)(pl.bx[b] - (pl.bx[s] + e.off[0]));
                                 dy = (float)(pl.by[b] - (pl.by[s] + e.off[1]));
                                 dz = (float)(pl.bz[b] - (pl.bz[s] + e.off[2]));
+                            } else if (mode == EMUL_P2P_TWOFLOAT) {
+                                const double rx = pl.bx[s] - cs.x[0], ry = pl.by[s] - cs.x[1], rz = pl.bz[s] - cs.x[2];
+                                const float shx = (float)rx, shy = (float)ry, shz = (float)rz;
+                                const float slx = (float)(rx - shx), sly = (float)(ry - shy), slz = (float)(rz - shz);
+                                dx = (xh - shx) + (xl - slx);
+                                dy = (yh - shy) + (yl - sly);
+                                dz = (zh - shz) + (zl - slz);
                             } else {
                                 dx = xo - (float)(pl.bx[s] - cs.x[0]);
                                 dy = yo - (float)(pl.by[s] - cs.x[1]);
@@ -1168,7 +1203,7 @@ static void stage_p2p_emul(Plan& pl, const double* q, double* p, double* gx, dou
                             const float ir = r2 > 0 ? 1.0f / std::sqrt(r2) : 0.0f;
                             float qi = (float)q[s] * ir;
                             const float qi3 = qi * (ir * ir);
-                            if (mode == EMUL_P2P_CELLREL) {
+                            if (mode == EMUL_P2P_CELLREL || mode == EMUL_P2P_TWOFLOAT) {
                                 fp += qi; fx += qi3 * dx; fy += qi3 * dy; fz += qi3 * dz;
                             } else {
                                 ap += qi; ax += (double)(qi3 * dx); ay += (double)(qi3 * dy); az += (double)(qi3 * dz);
@@ -1265,7 +1300,7 @@ int orc_set_option(void* h, const char* name, int v) {
     if (!strcmp(name, "emulate")) { pl->emul_mask = v; return 0; }
     if (!strcmp(name, "emulate_kind")) { pl->emul_kind = v; return v >= EMUL_F32 && v <= EMUL_BF16X6 ? 0 : -1; }
     if (!strcmp(name, "pinv_rank")) { pl->pinv_rank = v; return 0; }
-    if (!strcmp(name, "emulate_p2p")) { pl->emul_p2p = v; return v >= EMUL_P2P_OFF && v <= EMUL_P2P_DISP ? 0 : -1; }
+    if (!strcmp(name, "emulate_p2p")) { pl->emul_p2p = v; return v >= EMUL_P2P_OFF && v <= EMUL_P2P_TWOFLOAT ? 0 : -1; }
     return -1;
 }
 

@@ -110,7 +110,7 @@ class OracleFMM:
     EMUL_M2L = 128   # float32 M2L of the tree parents (up transform, 8x8 Hadamard, down transform)
     EMUL_F32, EMUL_BF16X3, EMUL_BF16X6 = 0, 1, 2
     # Forms of the float32 near field (orc_set_option "emulate_p2p"), see stage_p2p_emul.
-    EMUL_P2P_OFF, EMUL_P2P_CELLREL, EMUL_P2P_CELLREL_F64SUM, EMUL_P2P_DISP = 0, 1, 2, 3
+    EMUL_P2P_OFF, EMUL_P2P_CELLREL, EMUL_P2P_CELLREL_F64SUM, EMUL_P2P_DISP, EMUL_P2P_TWOFLOAT = 0, 1, 2, 3, 4
 
     def set_emulation(self, mask, kind=0, pinv_rank=0):
         """Run the stages in `mask` in the emulated arithmetic `kind` (float32, or split bfloat16

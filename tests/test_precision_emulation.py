@@ -18,7 +18,8 @@ and with 3 pieces and 6 products, each product exact and summed in float32.  Fin
   downward pass loses 2e-5 in the gradient of a periodic system;
 * the float32 near field's gradient error (4e-6) is the rounding of cell-relative coordinates to
   float32, not its float32 sums: a perfectly compensated sum leaves it unchanged, a displacement
-  formed in float64 and rounded once brings it to 1e-7;
+  formed in float64 and rounded once brings it to 1e-7, and so do two-float (hi + lo) coordinates
+  with float32 arithmetic throughout;
 * with the float32 M2L emulated as well (EMUL_M2L), the whole mixed path stays within its bounds,
   its gradient error is the P2P's and its potential error the far field's.
 """
@@ -165,16 +166,20 @@ def test_float32_near_field_error_is_the_coordinate_rounding(oracle_mod):
     f, _, q = _case(oracle_mod, 6, 1, 1, n=32768, seed=7)
     p0, g0 = f.evaluate(q)
     err = {}
-    for form in (O.EMUL_P2P_CELLREL, O.EMUL_P2P_CELLREL_F64SUM, O.EMUL_P2P_DISP):
+    for form in (O.EMUL_P2P_CELLREL, O.EMUL_P2P_CELLREL_F64SUM, O.EMUL_P2P_DISP, O.EMUL_P2P_TWOFLOAT):
         f.set_p2p_emulation(form)
         p, g = f.evaluate(q)
         err[form] = (_rel(p, p0), _rel(g, g0))
     f.set_p2p_emulation(O.EMUL_P2P_OFF)
     print("float32 near field, phi / grad vs float64: cell-relative coordinates {:.2e} / {:.2e}; the same with "
-          "float64 sums {:.2e} / {:.2e}; float64 displacement rounded once {:.2e} / {:.2e}".format(
-              *err[O.EMUL_P2P_CELLREL], *err[O.EMUL_P2P_CELLREL_F64SUM], *err[O.EMUL_P2P_DISP]))
+          "float64 sums {:.2e} / {:.2e}; float64 displacement rounded once {:.2e} / {:.2e}; two-float coordinates "
+          "{:.2e} / {:.2e}".format(*err[O.EMUL_P2P_CELLREL], *err[O.EMUL_P2P_CELLREL_F64SUM], *err[O.EMUL_P2P_DISP],
+                                   *err[O.EMUL_P2P_TWOFLOAT]))
     cell, summed, disp = err[O.EMUL_P2P_CELLREL][1], err[O.EMUL_P2P_CELLREL_F64SUM][1], err[O.EMUL_P2P_DISP][1]
     assert cell > 1e-6 and abs(summed - cell) < 0.1 * cell and disp < 0.1 * cell
+    # hi + lo float32 coordinates (two-sum per target and list entry, two differences per pair) recover
+    # the displacement's accuracy with float32 arithmetic throughout
+    assert err[O.EMUL_P2P_TWOFLOAT][1] < 0.05 * cell and err[O.EMUL_P2P_TWOFLOAT][0] < err[O.EMUL_P2P_CELLREL][0]
 
 
 def test_whole_mixed_path_emulated(oracle_mod):
